@@ -1,0 +1,159 @@
+"""Step driver of the lid-driven cavity: the CController / CManager caller
+surface of the reference, on one GPU.
+
+* ``cavity_parameters`` maps (edge, lid velocity, Re) to the CLbmSolver
+  quantities with domainLength = domainSize (dx = 1) and timestep = 1, so the
+  dimensionless lid velocity is u_lid and tau = 0.5 (6 nu + 1) with
+  nu = u_lid * N / Re (CLbmSolver.cpp:144-281; Re as in :284, N the largest
+  interior extent).
+* ``CavityController`` owns one or more subdomain solvers laid out like
+  CManager's rank grid (x fastest, CManager.cpp:34-58): OBSTACLE on global
+  faces, GHOST_LAYER between subdomains (:91-104), the lid on the top layer of
+  the top subdomains (:206-208, CController.cpp:1313-1345).
+* ``computeNextStep`` alternates alpha / beta by step parity
+  (CController.cpp:1278-1287). After each half-step the subdomains exchange
+  faces on the device in the reference order x, then y, then z
+  (CController.cpp:369-570 after alpha, 807-1008 after beta), with
+  ``lbm_exchange_face`` standing in for the MPI send/recv pair.
+* ``run`` times ``loops`` steps and reports the reference's benchmark figures
+  (CController.cpp:1382-1490): GLUPS over interior cells and its
+  "bandwidth" = GLUPS * (19 * 2 * sizeof(T) + sizeof(Flag) [+ 4 * sizeof(T)]).
+"""
+from __future__ import annotations
+
+import itertools
+import time
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+from .solver import LbmSolverHIP, compute_parameters, host_dtype, precision_code, _stream
+
+
+def cavity_parameters(size, u_lid, reynolds, precision="f32", max_velocity=0.1,
+                      max_acceleration=0.1):
+    """CLbmSolver-derived parameters of a cavity with dx = 1 and dt = 1."""
+    n = max(size)
+    viscosity = float(u_lid) * float(n) / float(reynolds)
+    return compute_parameters(length_x=size[0], size_x=size[0], length_max=n,
+                              velocity=(u_lid, 0.0, 0.0), acceleration=(0.0, 0.0, 0.0),
+                              viscosity=viscosity, timestep=1.0, max_velocity=max_velocity,
+                              max_acceleration=max_acceleration, precision=precision)
+
+
+class CavityController:
+    """Global cavity of ``size`` interior cells split into ``split`` subdomains."""
+
+    def __init__(self, size, u_lid=0.05, reynolds=100.0, precision="f32",
+                 store_outputs=True, split=(1, 1, 1), device=0, params=None, stream=None):
+        self.size = tuple(int(v) for v in size)
+        self.split = tuple(int(v) for v in split)
+        for a in range(3):
+            if self.size[a] % self.split[a]:
+                raise ValueError("domain size must be divisible by the split")
+        self.sub = tuple(self.size[a] // self.split[a] for a in range(3))
+        self.precision = precision_code(precision)
+        self.dtype = host_dtype(precision)
+        self.params = params or cavity_parameters(self.size, u_lid, reynolds, precision)
+        self.inv_tau = self.params["inv_tau"]
+        self.lid = self.params["velocity_dimless"][0]
+        self.store_outputs = bool(store_outputs)
+        self.stream = stream
+        self.simulationStepCounter = 0
+        self.solvers = {}
+        px, py, pz = self.split
+        for k, j, i in itertools.product(range(pz), range(py), range(px)):
+            bc = [_lib.GHOST_LAYER] * 6
+            if i == 0:
+                bc[0] = _lib.OBSTACLE
+            if i == px - 1:
+                bc[1] = _lib.OBSTACLE
+            if j == 0:
+                bc[2] = _lib.OBSTACLE
+            if j == py - 1:
+                bc[3] = _lib.OBSTACLE
+            if k == 0:
+                bc[4] = _lib.OBSTACLE
+            if k == pz - 1:
+                bc[5] = _lib.OBSTACLE
+            s = LbmSolverHIP(self.sub, bc, precision, self.inv_tau, self.lid,
+                             self.params["acceleration_dimless"], self.store_outputs,
+                             self.store_outputs, device)
+            if j == py - 1:
+                s.setDrivenCavitySzenario()
+            self.solvers[(i, j, k)] = s
+
+    def close(self):
+        for s in self.solvers.values():
+            s.close()
+        self.solvers = {}
+
+    # -- stepping ---------------------------------------------------------
+    def _exchange(self, phase: int):
+        L = lib()
+        st = _stream(self.stream)
+        for axis in range(3):
+            lo_dir, hi_dir = 2 * axis, 2 * axis + 1
+            for key, a in self.solvers.items():
+                nb = list(key)
+                nb[axis] += 1
+                b = self.solvers.get(tuple(nb))
+                if b is None:
+                    continue
+                check(L.lbm_exchange_face(a.handle, hi_dir, phase, b.handle, st))
+                check(L.lbm_exchange_face(b.handle, lo_dir, phase, a.handle, st))
+
+    def stepAlpha(self):
+        for s in self.solvers.values():
+            s.simulationStepAlpha(stream=self.stream)
+        if len(self.solvers) > 1:
+            self._exchange(0)
+
+    def stepBeta(self):
+        for s in self.solvers.values():
+            s.simulationStepBeta(stream=self.stream)
+        if len(self.solvers) > 1:
+            self._exchange(1)
+
+    def computeNextStep(self):
+        if self.simulationStepCounter & 1:
+            self.stepBeta()
+        else:
+            self.stepAlpha()
+        self.simulationStepCounter += 1
+
+    def sync(self):
+        for s in self.solvers.values():
+            s.sync(self.stream)
+            break
+
+    def run(self, loops: int) -> dict:
+        self.sync()
+        t0 = time.perf_counter()
+        for _ in range(loops):
+            self.computeNextStep()
+        self.sync()
+        elapsed = time.perf_counter() - t0
+        cells = int(np.prod(self.size))
+        t = self.dtype.itemsize
+        used = 19 * 2 * t + 4 + (4 * t if self.store_outputs else 0)
+        glups = loops * cells / elapsed * 1e-9
+        return dict(loops=loops, time_s=elapsed, iterations_per_s=loops / elapsed,
+                    glups=glups, mlups=glups * 1e3, bandwidth_gbs=glups * used)
+
+    # -- extraction -----------------------------------------------------------
+    def _gather(self, getter, ncomp):
+        out = np.empty((ncomp,) + self.size[::-1], dtype=self.dtype)
+        sx, sy, sz = self.sub
+        for (i, j, k), s in self.solvers.items():
+            part = getattr(s, getter)().reshape(ncomp, sz, sy, sx)
+            out[:, k * sz:(k + 1) * sz, j * sy:(j + 1) * sy, i * sx:(i + 1) * sx] = part
+        return out
+
+    def getVelocities(self) -> np.ndarray:
+        """Interior velocities [3][z][y][x] of the whole cavity."""
+        return self._gather("getVelocities", 3)
+
+    def getDensities(self) -> np.ndarray:
+        return self._gather("getDensities", 1)[0]

@@ -1,0 +1,775 @@
+// C ABI of the HIP solver: argument checks, device memory, host<->device
+// staging, and dispatch to the kernels. No entry point calls exit(); errors
+// come back as lbm_status codes with a per-thread message.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+
+#include "lbm_abi.h"
+#include "lbm_d3q19.hpp"
+#include "lbm_internal.hpp"
+
+namespace lbm {
+
+static thread_local std::string g_error;
+
+void set_error(const std::string& msg) { g_error = msg; }
+
+int hip_fail(hipError_t e, const char* what)
+{
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return LBM_ERR_HIP;
+}
+
+}  // namespace lbm
+
+using namespace lbm;
+
+#define LBM_HIP(call, what)                       \
+    do {                                          \
+        hipError_t e_ = (call);                   \
+        if (e_ != hipSuccess)                     \
+            return hip_fail(e_, what);            \
+    } while (0)
+
+namespace {
+
+bool valid_flag(int f)
+{
+    return f == LBM_OBSTACLE || f == LBM_FLUID || f == LBM_VELOCITY_INJECTION ||
+           f == LBM_GHOST_LAYER;
+}
+
+int element_bytes(int precision)
+{
+    switch (precision) {
+    case LBM_F32: return 4;
+    case LBM_F64: return 8;
+    case LBM_F16S: return 2;
+    case LBM_BF16S: return 2;
+    default: return 0;
+    }
+}
+
+// Resolves a (possibly NULL) box. whole: 0 -> full halo-inclusive domain,
+// 1 -> interior. Reference semantic: assert origin >= 0, size > 0,
+// origin + size <= size_h (CLbmSolverCPU.cpp:145-149).
+int resolve_box(const lbm_solver* s, const int* origin, const int* size, int whole, Box* b)
+{
+    if (!s) {
+        set_error("solver is NULL");
+        return LBM_ERR_ARG;
+    }
+    if (!origin || !size) {
+        if (whole == 0)
+            *b = Box{0, 0, 0, s->size_h[0], s->size_h[1], s->size_h[2]};
+        else
+            *b = Box{1, 1, 1, s->p.size[0], s->p.size[1], s->p.size[2]};
+        return LBM_OK;
+    }
+    for (int i = 0; i < 3; i++) {
+        if (origin[i] < 0 || size[i] <= 0 || origin[i] + size[i] > s->size_h[i]) {
+            char buf[256];
+            snprintf(buf, sizeof buf,
+                     "box origin (%d,%d,%d) size (%d,%d,%d) outside domain with halo (%d,%d,%d)",
+                     origin[0], origin[1], origin[2], size[0], size[1], size[2], s->size_h[0],
+                     s->size_h[1], s->size_h[2]);
+            set_error(buf);
+            return LBM_ERR_BOX;
+        }
+    }
+    *b = Box{origin[0], origin[1], origin[2], size[0], size[1], size[2]};
+    return LBM_OK;
+}
+
+int set_device(const lbm_solver* s)
+{
+    int cur = -1;
+    LBM_HIP(hipGetDevice(&cur), "hipGetDevice");
+    if (cur != s->p.device)
+        LBM_HIP(hipSetDevice(s->p.device), "hipSetDevice");
+    return LBM_OK;
+}
+
+int ensure_staging(lbm_solver* s, size_t bytes)
+{
+    if (s->staging_bytes >= bytes)
+        return LBM_OK;
+    if (s->staging)
+        LBM_HIP(hipFree(s->staging), "hipFree(staging)");
+    s->staging = nullptr;
+    s->staging_bytes = 0;
+    if (hipMalloc(&s->staging, bytes) != hipSuccess) {
+        set_error("cannot allocate device staging buffer");
+        return LBM_ERR_NOMEM;
+    }
+    s->staging_bytes = bytes;
+    return LBM_OK;
+}
+
+// Host <-> device box copy of one field through the staging buffer.
+int copy_get(lbm_solver* s, int field, const Box& b, int ncomp, void* dst, int dst_on_device,
+             hipStream_t st)
+{
+    const size_t bytes = (size_t)b.cells() * ncomp * s->out_bytes;
+    if (dst_on_device) {
+        LBM_HIP(launch_box_get(s, field, b, dst, st), "box get kernel");
+        return LBM_OK;
+    }
+    int rc = ensure_staging(s, bytes);
+    if (rc)
+        return rc;
+    LBM_HIP(launch_box_get(s, field, b, s->staging, st), "box get kernel");
+    LBM_HIP(hipMemcpyAsync(dst, s->staging, bytes, hipMemcpyDeviceToHost, st), "D2H copy");
+    LBM_HIP(hipStreamSynchronize(st), "hipStreamSynchronize");
+    return LBM_OK;
+}
+
+int copy_set(lbm_solver* s, int field, const Box& b, int ncomp, const void* src,
+             int src_on_device, uint32_t mask, hipStream_t st)
+{
+    const size_t bytes = (size_t)b.cells() * ncomp * s->out_bytes;
+    if (src_on_device) {
+        LBM_HIP(launch_box_set(s, field, b, src, mask, st), "box set kernel");
+        return LBM_OK;
+    }
+    int rc = ensure_staging(s, bytes);
+    if (rc)
+        return rc;
+    LBM_HIP(hipMemcpyAsync(s->staging, src, bytes, hipMemcpyHostToDevice, st), "H2D copy");
+    LBM_HIP(launch_box_set(s, field, b, s->staging, mask, st), "box set kernel");
+    LBM_HIP(hipStreamSynchronize(st), "hipStreamSynchronize");
+    return LBM_OK;
+}
+
+// norm . c_l > 0 mask of setDensityDistributions(..., direction, ...)
+// (CLbmSolverCPU.cpp:564-619).
+uint32_t direction_mask(int direction)
+{
+    int norm[3] = {0, 0, 0};
+    switch (direction) {
+    case LBM_LEFT: norm[0] = 1; break;
+    case LBM_RIGHT: norm[0] = -1; break;
+    case LBM_BOTTOM: norm[1] = 1; break;
+    case LBM_TOP: norm[1] = -1; break;
+    case LBM_BACK: norm[2] = 1; break;
+    case LBM_FRONT: norm[2] = -1; break;
+    default: return 0;
+    }
+    uint32_t m = 0;
+    for (int l = 0; l < kQ; l++)
+        if (norm[0] * cx(l) + norm[1] * cy(l) + norm[2] * cz(l) > 0)
+            m |= 1u << l;
+    return m;
+}
+
+// The CLbmSolver constructor's time-step adaptation (CLbmSolver.cpp:75-281)
+// in arithmetic type T. REYNOLDS_DEFAULT and TAU_DEFAULT are double literals
+// in the reference (constants.h:26-27) and stay double here.
+template <class T>
+int compute_parameters_t(const lbm_physics* in, lbm_derived* out)
+{
+    constexpr double kReynoldsDefault = 1000.0;
+    constexpr double kTauDefault = 0.85;
+    const T eps = std::numeric_limits<T>::epsilon();
+    const T vel[3] = {(T)in->velocity[0], (T)in->velocity[1], (T)in->velocity[2]};
+    const T acc[3] = {(T)in->acceleration[0], (T)in->acceleration[1], (T)in->acceleration[2]};
+    auto len = [](const T v[3]) { return (T)std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); };
+    const T max_vel = (T)in->max_velocity, max_acc = (T)in->max_acceleration;
+    const T cell_length = (T)in->length_x / (T)in->size_x;
+    T viscosity = (T)in->viscosity;
+    T timestep = (T)in->timestep;
+    T vd[3], ad[3], visc_d = 0, tau = 0, old_timestep;
+    bool by_velocity = false, by_acceleration = false;
+
+    if (viscosity <= (T)0)
+        viscosity = (T)in->length_max * vel[0] / kReynoldsDefault;
+
+    for (int iter = 0;; iter++) {
+        if (iter >= 64) {
+            set_error("time-step adaptation did not settle");
+            return LBM_ERR_PARAMS;
+        }
+        T sc = timestep / cell_length;
+        for (int i = 0; i < 3; i++)
+            vd[i] = vel[i] * sc;
+        sc = (timestep * timestep) / cell_length;
+        for (int i = 0; i < 3; i++)
+            ad[i] = acc[i] * sc;
+        visc_d = viscosity * (timestep / (cell_length * cell_length));
+        if (len(vd) > max_vel + eps) {
+            by_velocity = true;
+            old_timestep = timestep;
+            timestep = (max_vel * cell_length) / len(vel);
+            continue;
+        }
+        if (len(ad) > max_acc + eps) {
+            by_acceleration = true;
+            old_timestep = timestep;
+            timestep = (T)std::sqrt((max_acc * cell_length) / len(acc));
+            continue;
+        }
+        tau = (T)0.5 * ((T)6 * visc_d + (T)1);
+        if (tau - eps < (T)0.51 || tau + eps > (T)1.9) {
+            old_timestep = timestep;
+            timestep = (cell_length * cell_length) * (((T)2 * kTauDefault - (T)1) / ((T)6 * viscosity));
+            if ((by_velocity || by_acceleration) && timestep > old_timestep) {
+                set_error("no valid timestep size satisfies viscosity, max velocity, "
+                          "max acceleration and tau limits");
+                return LBM_ERR_PARAMS;
+            }
+            continue;
+        }
+        break;
+    }
+    for (int i = 0; i < 3; i++) {
+        out->velocity_dimless[i] = vd[i];
+        out->acceleration_dimless[i] = ad[i];
+    }
+    out->viscosity = viscosity;
+    out->viscosity_dimless = visc_d;
+    out->tau = tau;
+    out->inv_tau = (T)1 / tau;
+    out->timestep = timestep;
+    return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_abi_version(void) { return 1; }
+
+const char* lbm_last_error(void) { return g_error.c_str(); }
+
+const char* lbm_status_string(int status)
+{
+    switch (status) {
+    case LBM_OK: return "ok";
+    case LBM_ERR_ARG: return "invalid argument";
+    case LBM_ERR_BOX: return "box outside the domain";
+    case LBM_ERR_HIP: return "HIP runtime error";
+    case LBM_ERR_NOMEM: return "out of device memory";
+    case LBM_ERR_PARAMS: return "no valid time step";
+    case LBM_ERR_STATE: return "invalid solver state";
+    case LBM_ERR_UNSUPPORTED: return "unsupported";
+    default: return "unknown status";
+    }
+}
+
+int lbm_compute_parameters(const lbm_physics* in, int precision, lbm_derived* out)
+{
+    if (!in || !out || in->size_x <= 0 || !(in->length_x > 0) || !(in->timestep > 0) ||
+        !(in->max_velocity > 0) || !(in->max_acceleration > 0)) {
+        set_error("lbm_compute_parameters: invalid physics input");
+        return LBM_ERR_ARG;
+    }
+    if (precision == LBM_F64)
+        return compute_parameters_t<double>(in, out);
+    return compute_parameters_t<float>(in, out);
+}
+
+int lbm_create(const lbm_params* params, lbm_solver** out)
+{
+    if (!params || !out) {
+        set_error("lbm_create: NULL argument");
+        return LBM_ERR_ARG;
+    }
+    *out = nullptr;
+    const lbm_params& p = *params;
+    const int eb = element_bytes(p.precision);
+    if (!eb) {
+        set_error("lbm_create: unknown precision");
+        return LBM_ERR_ARG;
+    }
+    for (int i = 0; i < 3; i++)
+        if (p.size[i] < 1) {
+            set_error("lbm_create: size must be >= 1 on every axis");
+            return LBM_ERR_ARG;
+        }
+    for (int i = 0; i < 6; i++)
+        if (!valid_flag(p.bc[i])) {
+            set_error("lbm_create: boundary condition is not a Flag value");
+            return LBM_ERR_ARG;
+        }
+    const int64_t nx = p.size[0] + 2, ny = p.size[1] + 2, nz = p.size[2] + 2;
+    const int64_t n = nx * ny * nz;
+    // 32-bit cell indices with room for the largest neighbour offset
+    if (n + nx * ny + nx + 1 >= (int64_t)INT32_MAX) {
+        set_error("lbm_create: domain too large for 32-bit cell indexing");
+        return LBM_ERR_ARG;
+    }
+    lbm_solver* s = new lbm_solver();
+    std::memset(s, 0, sizeof(*s));
+    s->p = p;
+    s->size_h[0] = (int)nx;
+    s->size_h[1] = (int)ny;
+    s->size_h[2] = (int)nz;
+    s->n = n;
+    s->stride = (n + 255) / 256 * 256;
+    s->elem_bytes = eb;
+    s->out_bytes = p.precision == LBM_F64 ? 8 : 4;
+    // round inv_tau / lid / gravity to the arithmetic type once
+    if (p.precision != LBM_F64) {
+        s->p.inv_tau = (float)p.inv_tau;
+        s->p.lid_velocity = (float)p.lid_velocity;
+        for (int i = 0; i < 3; i++)
+            s->p.gravitation[i] = (float)p.gravitation[i];
+    }
+    int rc = set_device(s);
+    if (rc) {
+        delete s;
+        return rc;
+    }
+    auto fail = [&](int code, const char* what) {
+        set_error(std::string("lbm_create: ") + what);
+        lbm_destroy(s);
+        return code;
+    };
+    if (hipMalloc(&s->dd, (size_t)kQ * s->stride * eb) != hipSuccess)
+        return fail(LBM_ERR_NOMEM, "cannot allocate populations");
+    if (hipMalloc((void**)&s->flags, (size_t)s->stride) != hipSuccess)
+        return fail(LBM_ERR_NOMEM, "cannot allocate flags");
+    if (p.store_densities && hipMalloc(&s->rho, (size_t)s->stride * s->out_bytes) != hipSuccess)
+        return fail(LBM_ERR_NOMEM, "cannot allocate densities");
+    if (p.store_velocities &&
+        hipMalloc(&s->u, (size_t)3 * s->stride * s->out_bytes) != hipSuccess)
+        return fail(LBM_ERR_NOMEM, "cannot allocate velocities");
+    if (hipMalloc((void**)&s->partials, (size_t)(kReduceBlocks + 1) * 5 * sizeof(double)) !=
+        hipSuccess)
+        return fail(LBM_ERR_NOMEM, "cannot allocate reduction buffer");
+    hipError_t e = launch_init(s, nullptr);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) {
+        hip_fail(e, "lbm_create: init kernel");
+        lbm_destroy(s);
+        return LBM_ERR_HIP;
+    }
+    *out = s;
+    return LBM_OK;
+}
+
+int lbm_destroy(lbm_solver* s)
+{
+    if (!s)
+        return LBM_OK;
+    int cur = -1;
+    (void)hipGetDevice(&cur);
+    if (cur != s->p.device)
+        (void)hipSetDevice(s->p.device);
+    (void)hipDeviceSynchronize();
+    void* bufs[] = {s->dd, s->flags, s->rho, s->u, s->staging, s->partials, s->residual_prev};
+    for (void* b : bufs)
+        if (b)
+            (void)hipFree(b);
+    delete s;
+    return LBM_OK;
+}
+
+int lbm_get_device_arrays(const lbm_solver* s, lbm_device_arrays* out)
+{
+    if (!s || !out) {
+        set_error("lbm_get_device_arrays: NULL argument");
+        return LBM_ERR_ARG;
+    }
+    out->dd = s->dd;
+    out->flags = s->flags;
+    out->densities = s->rho;
+    out->velocities = s->u;
+    out->cells = s->n;
+    out->plane_stride = s->stride;
+    for (int i = 0; i < 3; i++)
+        out->size_h[i] = s->size_h[i];
+    out->precision = s->p.precision;
+    out->element_bytes = s->elem_bytes;
+    return LBM_OK;
+}
+
+int lbm_simulation_step_alpha(lbm_solver* s, const int origin[3], const int size[3],
+                              hipStream_t stream)
+{
+    Box b;
+    int rc = resolve_box(s, origin, size, 0, &b);
+    if (rc)
+        return rc;
+    if ((rc = set_device(s)))
+        return rc;
+    LBM_HIP(launch_alpha(s, b, stream), "alpha kernel launch");
+    return LBM_OK;
+}
+
+int lbm_simulation_step_beta(lbm_solver* s, const int origin[3], const int size[3],
+                             hipStream_t stream)
+{
+    Box b;
+    int rc = resolve_box(s, origin, size, 0, &b);
+    if (rc)
+        return rc;
+    if ((rc = set_device(s)))
+        return rc;
+    LBM_HIP(launch_beta(s, b, stream), "beta kernel launch");
+    return LBM_OK;
+}
+
+int lbm_get_density_distributions(lbm_solver* s, const int origin[3], const int size[3],
+                                  void* dst, int dst_on_device, hipStream_t stream)
+{
+    Box b;
+    int rc = resolve_box(s, origin, size, 1, &b);
+    if (rc)
+        return rc;
+    if (!dst) {
+        set_error("destination is NULL");
+        return LBM_ERR_ARG;
+    }
+    if ((rc = set_device(s)))
+        return rc;
+    return copy_get(s, 0, b, kQ, dst, dst_on_device, stream);
+}
+
+int lbm_set_density_distributions(lbm_solver* s, const int origin[3], const int size[3],
+                                  const void* src, int src_on_device, hipStream_t stream)
+{
+    Box b;
+    int rc = resolve_box(s, origin, size, 1, &b);
+    if (rc)
+        return rc;
+    if (!src) {
+        set_error("source is NULL");
+        return LBM_ERR_ARG;
+    }
+    if ((rc = set_device(s)))
+        return rc;
+    return copy_set(s, 0, b, kQ, src, src_on_device, 0x7FFFFu, stream);
+}
+
+int lbm_set_density_distributions_dir(lbm_solver* s, const int origin[3], const int size[3],
+                                      int direction, const void* src, int src_on_device,
+                                      hipStream_t stream)
+{
+    Box b;
+    int rc = resolve_box(s, origin, size, 1, &b);
+    if (rc)
+        return rc;
+    const uint32_t mask = direction_mask(direction);
+    if (!mask || !src) {
+        set_error("invalid direction or NULL source");
+        return LBM_ERR_ARG;
+    }
+    if ((rc = set_device(s)))
+        return rc;
+    return copy_set(s, 0, b, kQ, src, src_on_device, mask, stream);
+}
+
+int lbm_get_flags(lbm_solver* s, const int origin[3], const int size[3], int32_t* dst)
+{
+    Box b;
+    int rc = resolve_box(s, origin, size, 1, &b);
+    if (rc)
+        return rc;
+    if (!dst) {
+        set_error("destination is NULL");
+        return LBM_ERR_ARG;
+    }
+    if ((rc = set_device(s)))
+        return rc;
+    const size_t bytes = (size_t)b.cells() * sizeof(int32_t);
+    if ((rc = ensure_staging(s, bytes)))
+        return rc;
+    LBM_HIP(launch_flags_get(s, b, (int32_t*)s->staging, nullptr), "flags get kernel");
+    LBM_HIP(hipMemcpy(dst, s->staging, bytes, hipMemcpyDeviceToHost), "D2H copy");
+    return LBM_OK;
+}
+
+int lbm_set_flags(lbm_solver* s, const int origin[3], const int size[3], const int32_t* src)
+{
+    Box b;
+    int rc = resolve_box(s, origin, size, 1, &b);
+    if (rc)
+        return rc;
+    if (!src) {
+        set_error("source is NULL");
+        return LBM_ERR_ARG;
+    }
+    const uint32_t cells = b.cells();
+    for (uint32_t i = 0; i < cells; i++)
+        if (!valid_flag(src[i])) {
+            set_error("setFlags: value is not one of OBSTACLE, FLUID, VELOCITY_INJECTION, "
+                      "GHOST_LAYER");
+            return LBM_ERR_ARG;
+        }
+    if ((rc = set_device(s)))
+        return rc;
+    const size_t bytes = (size_t)cells * sizeof(int32_t);
+    if ((rc = ensure_staging(s, bytes)))
+        return rc;
+    LBM_HIP(hipMemcpy(s->staging, src, bytes, hipMemcpyHostToDevice), "H2D copy");
+    LBM_HIP(launch_flags_set(s, b, (const int32_t*)s->staging, nullptr), "flags set kernel");
+    LBM_HIP(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
+    return LBM_OK;
+}
+
+static int field_get(lbm_solver* s, int field, const int origin[3], const int size[3],
+                     void* dst, int on_dev, hipStream_t st)
+{
+    Box b;
+    int rc = resolve_box(s, origin, size, 1, &b);
+    if (rc)
+        return rc;
+    if ((field == 1 && !s->u) || (field == 2 && !s->rho)) {
+        set_error("field is not stored by this solver (store_velocities/store_densities)");
+        return LBM_ERR_UNSUPPORTED;
+    }
+    if (!dst) {
+        set_error("destination is NULL");
+        return LBM_ERR_ARG;
+    }
+    if ((rc = set_device(s)))
+        return rc;
+    return copy_get(s, field, b, field == 1 ? 3 : 1, dst, on_dev, st);
+}
+
+static int field_set(lbm_solver* s, int field, const int origin[3], const int size[3],
+                     const void* src, int on_dev, hipStream_t st)
+{
+    Box b;
+    int rc = resolve_box(s, origin, size, 1, &b);
+    if (rc)
+        return rc;
+    if ((field == 1 && !s->u) || (field == 2 && !s->rho)) {
+        set_error("field is not stored by this solver (store_velocities/store_densities)");
+        return LBM_ERR_UNSUPPORTED;
+    }
+    if (!src) {
+        set_error("source is NULL");
+        return LBM_ERR_ARG;
+    }
+    if ((rc = set_device(s)))
+        return rc;
+    return copy_set(s, field, b, field == 1 ? 3 : 1, src, on_dev, 0x7u, st);
+}
+
+int lbm_get_velocities(lbm_solver* s, const int origin[3], const int size[3], void* dst,
+                       int dst_on_device, hipStream_t stream)
+{
+    return field_get(s, 1, origin, size, dst, dst_on_device, stream);
+}
+int lbm_set_velocities(lbm_solver* s, const int origin[3], const int size[3], const void* src,
+                       int src_on_device, hipStream_t stream)
+{
+    return field_set(s, 1, origin, size, src, src_on_device, stream);
+}
+int lbm_get_densities(lbm_solver* s, const int origin[3], const int size[3], void* dst,
+                      int dst_on_device, hipStream_t stream)
+{
+    return field_get(s, 2, origin, size, dst, dst_on_device, stream);
+}
+int lbm_set_densities(lbm_solver* s, const int origin[3], const int size[3], const void* src,
+                      int src_on_device, hipStream_t stream)
+{
+    return field_set(s, 2, origin, size, src, src_on_device, stream);
+}
+
+// CComm send/recv boxes, CManager.cpp:137-202.
+int lbm_face_box(const lbm_solver* s, int direction, int which, int origin[3], int size[3])
+{
+    if (!s || !origin || !size || direction < 0 || direction > 5 || which < 0 || which > 1) {
+        set_error("lbm_face_box: invalid argument");
+        return LBM_ERR_ARG;
+    }
+    const int axis = direction / 2;
+    const bool high = direction & 1;
+    for (int i = 0; i < 3; i++) {
+        origin[i] = 0;
+        size[i] = s->size_h[i];
+    }
+    size[axis] = 1;
+    if (which == 0)
+        origin[axis] = high ? s->size_h[axis] - 2 : 1;
+    else
+        origin[axis] = high ? s->size_h[axis] - 1 : 0;
+    return LBM_OK;
+}
+
+int64_t lbm_face_elements(const lbm_solver* s, int direction)
+{
+    if (!s || direction < 0 || direction > 5)
+        return 0;
+    const int axis = direction / 2;
+    int64_t cells = 1;
+    for (int i = 0; i < 3; i++)
+        if (i != axis)
+            cells *= s->size_h[i];
+    return cells * kQ;
+}
+
+int lbm_pack_face(lbm_solver* s, int direction, int phase, void* dev_buf, hipStream_t stream)
+{
+    int o[3], z[3];
+    if (phase < 0 || phase > 1 || !dev_buf) {
+        set_error("lbm_pack_face: invalid phase or NULL buffer");
+        return LBM_ERR_ARG;
+    }
+    int rc = lbm_face_box(s, direction, phase == 0 ? 0 : 1, o, z);
+    if (rc)
+        return rc;
+    if ((rc = set_device(s)))
+        return rc;
+    Box b{o[0], o[1], o[2], z[0], z[1], z[2]};
+    LBM_HIP(launch_box_get(s, 0, b, dev_buf, stream), "pack kernel");
+    return LBM_OK;
+}
+
+int lbm_unpack_face(lbm_solver* s, int direction, int phase, const void* dev_buf,
+                    hipStream_t stream)
+{
+    int o[3], z[3];
+    if (phase < 0 || phase > 1 || !dev_buf) {
+        set_error("lbm_unpack_face: invalid phase or NULL buffer");
+        return LBM_ERR_ARG;
+    }
+    int rc = lbm_face_box(s, direction, phase == 0 ? 1 : 0, o, z);
+    if (rc)
+        return rc;
+    if ((rc = set_device(s)))
+        return rc;
+    Box b{o[0], o[1], o[2], z[0], z[1], z[2]};
+    const uint32_t mask = phase == 0 ? 0x7FFFFu : direction_mask(direction);
+    LBM_HIP(launch_box_set(s, 0, b, dev_buf, mask, stream), "unpack kernel");
+    return LBM_OK;
+}
+
+int lbm_exchange_face(lbm_solver* from, int direction, int phase, lbm_solver* to,
+                      hipStream_t stream)
+{
+    if (!from || !to || direction < 0 || direction > 5 || phase < 0 || phase > 1) {
+        set_error("lbm_exchange_face: invalid argument");
+        return LBM_ERR_ARG;
+    }
+    if (from->p.device != to->p.device || from->p.precision != to->p.precision) {
+        set_error("lbm_exchange_face: solvers differ in device or precision");
+        return LBM_ERR_ARG;
+    }
+    const int axis = direction / 2;
+    for (int i = 0; i < 3; i++)
+        if (i != axis && from->size_h[i] != to->size_h[i]) {
+            set_error("lbm_exchange_face: face extents differ");
+            return LBM_ERR_ARG;
+        }
+    int rc = set_device(from);
+    if (rc)
+        return rc;
+    const size_t bytes = (size_t)lbm_face_elements(from, direction) * from->out_bytes;
+    if ((rc = ensure_staging(from, bytes)))
+        return rc;
+    if ((rc = lbm_pack_face(from, direction, phase, from->staging, stream)))
+        return rc;
+    return lbm_unpack_face(to, direction ^ 1, phase, from->staging, stream);
+}
+
+int lbm_device_alloc(int device, size_t bytes, void** out)
+{
+    if (!out || !bytes) {
+        set_error("lbm_device_alloc: invalid argument");
+        return LBM_ERR_ARG;
+    }
+    *out = nullptr;
+    int cur = -1;
+    LBM_HIP(hipGetDevice(&cur), "hipGetDevice");
+    if (cur != device)
+        LBM_HIP(hipSetDevice(device), "hipSetDevice");
+    if (hipMalloc(out, bytes) != hipSuccess) {
+        set_error("lbm_device_alloc: out of device memory");
+        return LBM_ERR_NOMEM;
+    }
+    return LBM_OK;
+}
+
+int lbm_device_free(int device, void* ptr)
+{
+    if (!ptr)
+        return LBM_OK;
+    int cur = -1;
+    LBM_HIP(hipGetDevice(&cur), "hipGetDevice");
+    if (cur != device)
+        LBM_HIP(hipSetDevice(device), "hipSetDevice");
+    LBM_HIP(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    LBM_HIP(hipFree(ptr), "hipFree");
+    return LBM_OK;
+}
+
+int lbm_reduce_box(lbm_solver* s, const int origin[3], const int size[3], double out[5],
+                   hipStream_t stream)
+{
+    Box b;
+    int rc = resolve_box(s, origin, size, 0, &b);
+    if (rc)
+        return rc;
+    if (!out) {
+        set_error("output is NULL");
+        return LBM_ERR_ARG;
+    }
+    if ((rc = set_device(s)))
+        return rc;
+    double* res = s->partials + kReduceBlocks * 5;
+    LBM_HIP(launch_reduce_box(s, b, s->partials, res, stream), "reduce kernel");
+    LBM_HIP(hipMemcpyAsync(out, res, 5 * sizeof(double), hipMemcpyDeviceToHost, stream),
+            "D2H copy");
+    LBM_HIP(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    return LBM_OK;
+}
+
+int lbm_residual(lbm_solver* s, int layout, double* residual, hipStream_t stream)
+{
+    if (!s || !residual || layout < 0 || layout > 1) {
+        set_error("lbm_residual: invalid argument");
+        return LBM_ERR_ARG;
+    }
+    int rc = set_device(s);
+    if (rc)
+        return rc;
+    if (!s->residual_prev) {
+        const size_t bytes = (size_t)3 * s->stride * s->out_bytes;
+        if (hipMalloc(&s->residual_prev, bytes) != hipSuccess) {
+            set_error("cannot allocate residual snapshot");
+            return LBM_ERR_NOMEM;
+        }
+        LBM_HIP(hipMemset(s->residual_prev, 0, bytes), "hipMemset");
+        s->residual_valid = 0;
+    }
+    double* res = s->partials + kReduceBlocks * 5;
+    double host[5];
+    LBM_HIP(launch_residual(s, layout, s->partials, res, stream), "residual kernel");
+    LBM_HIP(hipMemcpyAsync(host, res, 5 * sizeof(double), hipMemcpyDeviceToHost, stream),
+            "D2H copy");
+    LBM_HIP(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    if (!s->residual_valid) {
+        *residual = std::numeric_limits<double>::infinity();
+        s->residual_valid = 1;
+    } else {
+        *residual = host[1] > 0 ? std::sqrt(host[0] / host[1])
+                                : (host[0] > 0 ? std::numeric_limits<double>::infinity() : 0.0);
+    }
+    return LBM_OK;
+}
+
+int lbm_sync(lbm_solver* s, hipStream_t stream)
+{
+    if (!s) {
+        set_error("solver is NULL");
+        return LBM_ERR_ARG;
+    }
+    int rc = set_device(s);
+    if (rc)
+        return rc;
+    LBM_HIP(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    return LBM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,206 @@
+// Box copies between the strided device fields and dense [c][z][y][x]
+// buffers, on the device. One kernel launch moves every selected component
+// of a box (the reference issues one cudaMemcpy3D per lattice vector,
+// CLbmSolverGPU.cu:553-568). Used for halo pack/unpack, checkpointing and the
+// rho/u/flag extraction surface.
+#include "lbm_d3q19.hpp"
+#include "lbm_internal.hpp"
+#include "lbm_storage.hpp"
+
+namespace lbm {
+
+constexpr int kCopyBlock = 256;
+
+struct BoxArgs {
+    int64_t stride;
+    int32_t nx, nxy;
+    int32_t ox, oy, oz;
+    uint32_t cells;
+    FastDiv div_bx, div_by;
+    uint32_t mask;
+};
+
+__device__ __forceinline__ uint32_t box_index(const BoxArgs& a, uint32_t gid)
+{
+    const uint32_t t = a.div_bx.div(gid);
+    const uint32_t x = gid - t * a.div_bx.d;
+    const uint32_t z = a.div_by.div(t);
+    const uint32_t y = t - z * a.div_by.d;
+    return (uint32_t)((a.oz + (int32_t)z) * a.nxy + (a.oy + (int32_t)y) * a.nx + a.ox + (int32_t)x);
+}
+
+// populations: grid.y = lattice vector; shifted storage is converted to f
+template <class P>
+__global__ __launch_bounds__(kCopyBlock) void k_get_dd(const typename P::S* __restrict__ dd,
+                                                       typename P::C* __restrict__ dst,
+                                                       BoxArgs a)
+{
+    const int l = blockIdx.y;
+    const uint32_t gid = blockIdx.x * kCopyBlock + threadIdx.x;
+    if (gid >= a.cells)
+        return;
+    const uint32_t id = box_index(a, gid);
+    dst[(int64_t)l * a.cells + gid] = P::load(dd[(int64_t)l * a.stride + id], l);
+}
+
+template <class P>
+__global__ __launch_bounds__(kCopyBlock) void k_set_dd(typename P::S* __restrict__ dd,
+                                                       const typename P::C* __restrict__ src,
+                                                       BoxArgs a)
+{
+    const int l = blockIdx.y;
+    if (!((a.mask >> l) & 1u))
+        return;
+    const uint32_t gid = blockIdx.x * kCopyBlock + threadIdx.x;
+    if (gid >= a.cells)
+        return;
+    const uint32_t id = box_index(a, gid);
+    dd[(int64_t)l * a.stride + id] = P::store(src[(int64_t)l * a.cells + gid], l);
+}
+
+// rho / u fields (compute type, no conversion)
+template <class C>
+__global__ __launch_bounds__(kCopyBlock) void k_get_field(const C* __restrict__ f,
+                                                          C* __restrict__ dst, BoxArgs a)
+{
+    const int c = blockIdx.y;
+    const uint32_t gid = blockIdx.x * kCopyBlock + threadIdx.x;
+    if (gid >= a.cells)
+        return;
+    dst[(int64_t)c * a.cells + gid] = f[(int64_t)c * a.stride + box_index(a, gid)];
+}
+
+template <class C>
+__global__ __launch_bounds__(kCopyBlock) void k_set_field(C* __restrict__ f,
+                                                          const C* __restrict__ src, BoxArgs a)
+{
+    const int c = blockIdx.y;
+    const uint32_t gid = blockIdx.x * kCopyBlock + threadIdx.x;
+    if (gid >= a.cells)
+        return;
+    f[(int64_t)c * a.stride + box_index(a, gid)] = src[(int64_t)c * a.cells + gid];
+}
+
+__global__ __launch_bounds__(kCopyBlock) void k_get_flags(const uint8_t* __restrict__ f,
+                                                          int32_t* __restrict__ dst, BoxArgs a)
+{
+    const uint32_t gid = blockIdx.x * kCopyBlock + threadIdx.x;
+    if (gid >= a.cells)
+        return;
+    dst[gid] = f[box_index(a, gid)];
+}
+
+__global__ __launch_bounds__(kCopyBlock) void k_set_flags(uint8_t* __restrict__ f,
+                                                          const int32_t* __restrict__ src,
+                                                          BoxArgs a)
+{
+    const uint32_t gid = blockIdx.x * kCopyBlock + threadIdx.x;
+    if (gid >= a.cells)
+        return;
+    f[box_index(a, gid)] = (uint8_t)src[gid];
+}
+
+static BoxArgs box_args(const lbm_solver* s, const Box& b, uint32_t mask)
+{
+    BoxArgs a;
+    a.stride = s->stride;
+    a.nx = s->size_h[0];
+    a.nxy = s->size_h[0] * s->size_h[1];
+    a.ox = b.ox;
+    a.oy = b.oy;
+    a.oz = b.oz;
+    a.cells = b.cells();
+    a.div_bx = FastDiv((uint32_t)b.bx);
+    a.div_by = FastDiv((uint32_t)b.by);
+    a.mask = mask;
+    return a;
+}
+
+template <class P>
+static hipError_t get_dd_t(const lbm_solver* s, const Box& b, void* dst, hipStream_t st)
+{
+    BoxArgs a = box_args(s, b, 0x7FFFFu);
+    dim3 grid((a.cells + kCopyBlock - 1) / kCopyBlock, kQ);
+    hipLaunchKernelGGL(k_get_dd<P>, grid, dim3(kCopyBlock), 0, st,
+                       (const typename P::S*)s->dd, (typename P::C*)dst, a);
+    return hipGetLastError();
+}
+
+template <class P>
+static hipError_t set_dd_t(const lbm_solver* s, const Box& b, const void* src, uint32_t mask,
+                           hipStream_t st)
+{
+    BoxArgs a = box_args(s, b, mask);
+    dim3 grid((a.cells + kCopyBlock - 1) / kCopyBlock, kQ);
+    hipLaunchKernelGGL(k_set_dd<P>, grid, dim3(kCopyBlock), 0, st, (typename P::S*)s->dd,
+                       (const typename P::C*)src, a);
+    return hipGetLastError();
+}
+
+template <class C>
+static hipError_t field_t(const lbm_solver* s, int field, const Box& b, void* buf, bool get,
+                          hipStream_t st)
+{
+    BoxArgs a = box_args(s, b, 0);
+    const int ncomp = field == 1 ? 3 : 1;
+    C* f = (C*)(field == 1 ? s->u : s->rho);
+    dim3 grid((a.cells + kCopyBlock - 1) / kCopyBlock, ncomp);
+    if (get)
+        hipLaunchKernelGGL(k_get_field<C>, grid, dim3(kCopyBlock), 0, st, f, (C*)buf, a);
+    else
+        hipLaunchKernelGGL(k_set_field<C>, grid, dim3(kCopyBlock), 0, st, f, (const C*)buf, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_box_get(const lbm_solver* s, int field, const Box& b, void* dst,
+                          hipStream_t st)
+{
+    if (field == 0) {
+        switch (s->p.precision) {
+        case LBM_F32: return get_dd_t<StoreF32>(s, b, dst, st);
+        case LBM_F64: return get_dd_t<StoreF64>(s, b, dst, st);
+        case LBM_F16S: return get_dd_t<StoreF16S>(s, b, dst, st);
+        case LBM_BF16S: return get_dd_t<StoreBF16S>(s, b, dst, st);
+        default: return hipErrorInvalidValue;
+        }
+    }
+    if (s->p.precision == LBM_F64)
+        return field_t<double>(s, field, b, dst, true, st);
+    return field_t<float>(s, field, b, dst, true, st);
+}
+
+hipError_t launch_box_set(const lbm_solver* s, int field, const Box& b, const void* src,
+                          uint32_t mask, hipStream_t st)
+{
+    if (field == 0) {
+        switch (s->p.precision) {
+        case LBM_F32: return set_dd_t<StoreF32>(s, b, src, mask, st);
+        case LBM_F64: return set_dd_t<StoreF64>(s, b, src, mask, st);
+        case LBM_F16S: return set_dd_t<StoreF16S>(s, b, src, mask, st);
+        case LBM_BF16S: return set_dd_t<StoreBF16S>(s, b, src, mask, st);
+        default: return hipErrorInvalidValue;
+        }
+    }
+    if (s->p.precision == LBM_F64)
+        return field_t<double>(s, field, b, const_cast<void*>(src), false, st);
+    return field_t<float>(s, field, b, const_cast<void*>(src), false, st);
+}
+
+hipError_t launch_flags_get(const lbm_solver* s, const Box& b, int32_t* dst, hipStream_t st)
+{
+    BoxArgs a = box_args(s, b, 0);
+    hipLaunchKernelGGL(k_get_flags, dim3((a.cells + kCopyBlock - 1) / kCopyBlock),
+                       dim3(kCopyBlock), 0, st, s->flags, dst, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_flags_set(const lbm_solver* s, const Box& b, const int32_t* src,
+                            hipStream_t st)
+{
+    BoxArgs a = box_args(s, b, 0);
+    hipLaunchKernelGGL(k_set_flags, dim3((a.cells + kCopyBlock - 1) / kCopyBlock),
+                       dim3(kCopyBlock), 0, st, s->flags, src, a);
+    return hipGetLastError();
+}
+
+}  // namespace lbm

@@ -1,0 +1,103 @@
+// Internal declarations of the HIP solver (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "lbm_abi.h"
+
+namespace lbm {
+
+// Unsigned 32-bit division by a runtime constant: q = (umulhi(n, m) + n) >> s,
+// exact for every 32-bit n (round-up multiplier, Granlund & Montgomery 1994).
+struct FastDiv {
+    uint32_t d = 1, m = 0, s = 0;
+    FastDiv() = default;
+    explicit FastDiv(uint32_t div) : d(div)
+    {
+        uint32_t l = 0;
+        while (l < 32 && (1ull << l) < div)
+            l++;
+        s = l;
+        m = (uint32_t)((((1ull << l) - div) << 32) / div + 1);
+        if (div == 1)
+            m = 0, s = 0;
+    }
+    __device__ __forceinline__ uint32_t div(uint32_t n) const
+    {
+        return (uint32_t)(((uint64_t)__umulhi(n, m) + n) >> s);
+    }
+};
+
+// A box of cells in halo coordinates.
+struct Box {
+    int ox, oy, oz;
+    int bx, by, bz;
+    uint32_t cells() const { return (uint32_t)bx * (uint32_t)by * (uint32_t)bz; }
+};
+
+// Everything a step kernel needs, passed by value.
+template <class C>
+struct StepArgs {
+    void* dd;
+    const uint8_t* flags;
+    C* rho;
+    C* u;
+    int64_t stride;   // elements between planes
+    int32_t n;        // cells with halo
+    int32_t nx, nxy;  // row and slab strides
+    int32_t ox, oy, oz;
+    uint32_t cells;   // cells of the box
+    FastDiv div_bx, div_by;
+    C inv_tau, lid, g0, g1, g2;
+    int store_rho, store_u;
+};
+
+}  // namespace lbm
+
+struct lbm_solver {
+    lbm_params p;
+    int size_h[3];
+    int64_t n;         // cells with halo
+    int64_t stride;    // plane stride in elements
+    int elem_bytes;    // bytes per stored population
+    int out_bytes;     // bytes per rho/u element (4 or 8)
+    void* dd;
+    uint8_t* flags;
+    void* rho;
+    void* u;
+    void* staging;     // device staging for host copies
+    size_t staging_bytes;
+    double* partials;  // reduction partials (kReduceBlocks * 5) followed by 5 results
+    void* residual_prev;  // momentum saved by lbm_residual, 3 planes of the compute type
+    int residual_valid;
+};
+
+namespace lbm {
+
+void set_error(const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+hipError_t launch_init(const lbm_solver* s, hipStream_t st);
+hipError_t launch_alpha(const lbm_solver* s, const Box& b, hipStream_t st);
+hipError_t launch_beta(const lbm_solver* s, const Box& b, hipStream_t st);
+
+// field: 0 populations (19 components, masked), 1 velocities (3), 2 densities (1)
+hipError_t launch_box_get(const lbm_solver* s, int field, const Box& b, void* dst,
+                          hipStream_t st);
+hipError_t launch_box_set(const lbm_solver* s, int field, const Box& b, const void* src,
+                          uint32_t mask, hipStream_t st);
+hipError_t launch_flags_get(const lbm_solver* s, const Box& b, int32_t* dst, hipStream_t st);
+hipError_t launch_flags_set(const lbm_solver* s, const Box& b, const int32_t* src,
+                            hipStream_t st);
+
+hipError_t launch_reduce_box(const lbm_solver* s, const Box& b, double* partials,
+                             double* out_dev, hipStream_t st);
+hipError_t launch_residual(const lbm_solver* s, int layout, double* partials, double* out_dev,
+                           hipStream_t st);
+
+constexpr int kReduceBlocks = 1024;
+
+}  // namespace lbm

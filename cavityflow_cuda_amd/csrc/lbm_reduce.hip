@@ -1,0 +1,181 @@
+// FP64 reductions over the population field: total mass / momentum of a box
+// and the steady-state residual ||u - u_prev||_2 / ||u||_2. Two passes with a
+// fixed block count and a fixed final order, so results are deterministic.
+#include "lbm_d3q19.hpp"
+#include "lbm_internal.hpp"
+#include "lbm_storage.hpp"
+
+namespace lbm {
+
+constexpr int kRedBlock = 256;
+constexpr int kRedVals = 5;
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Block sum of kRedVals doubles into partials[blockIdx.x * kRedVals + k].
+__device__ __forceinline__ void block_store(double (&acc)[kRedVals], double* partials)
+{
+    __shared__ double lds[kRedBlock / 64][kRedVals];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < kRedVals; k++) {
+        const double v = wave_sum(acc[k]);
+        if (lane == 0)
+            lds[wave][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kRedVals) {
+        double t = 0.0;
+        for (int w = 0; w < kRedBlock / 64; w++)
+            t += lds[w][threadIdx.x];
+        partials[blockIdx.x * kRedVals + threadIdx.x] = t;
+    }
+}
+
+struct RedArgs {
+    int64_t stride;
+    int32_t nx, nxy;
+    int32_t ox, oy, oz;
+    uint32_t cells;
+    FastDiv div_bx, div_by;
+};
+
+template <class P>
+__global__ __launch_bounds__(kRedBlock) void k_reduce_box(const typename P::S* __restrict__ dd,
+                                                          RedArgs a, double* partials)
+{
+    double acc[kRedVals] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (uint32_t gid = blockIdx.x * kRedBlock + threadIdx.x; gid < a.cells;
+         gid += gridDim.x * kRedBlock) {
+        const uint32_t t = a.div_bx.div(gid);
+        const uint32_t x = gid - t * a.div_bx.d;
+        const uint32_t z = a.div_by.div(t);
+        const uint32_t y = t - z * a.div_by.d;
+        const uint32_t id = (uint32_t)((a.oz + (int32_t)z) * a.nxy + (a.oy + (int32_t)y) * a.nx +
+                                       a.ox + (int32_t)x);
+#pragma unroll
+        for (int l = 0; l < kQ; l++) {
+            const double f = (double)P::load(dd[(int64_t)l * a.stride + id], l);
+            acc[0] += f;
+            acc[1] += cx(l) * f;
+            acc[2] += cy(l) * f;
+            acc[3] += cz(l) * f;
+        }
+        acc[4] += 1.0;
+    }
+    block_store(acc, partials);
+}
+
+// Residual over FLUID cells. layout 0: slot l holds direction l (after beta);
+// layout 1: slot l holds direction opp(l) (after alpha).
+template <class P>
+__global__ __launch_bounds__(kRedBlock) void k_residual(const typename P::S* __restrict__ dd,
+                                                        const uint8_t* __restrict__ flags,
+                                                        typename P::C* __restrict__ prev,
+                                                        int64_t stride, uint32_t n, int layout,
+                                                        double* partials)
+{
+    using C = typename P::C;
+    double acc[kRedVals] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (uint32_t id = blockIdx.x * kRedBlock + threadIdx.x; id < n;
+         id += gridDim.x * kRedBlock) {
+        if (flags[id] != kFluid)
+            continue;
+        C d[kQ];
+#pragma unroll
+        for (int l = 0; l < kQ; l++) {
+            const C v = P::load(dd[(int64_t)l * stride + id], l);
+            if (layout)
+                d[opp(l)] = v;
+            else
+                d[l] = v;
+        }
+        C rho, ux, uy, uz;
+        moments(d, rho, ux, uy, uz);
+        const double dx = (double)ux - (double)prev[id];
+        const double dy = (double)uy - (double)prev[stride + id];
+        const double dz = (double)uz - (double)prev[2 * stride + id];
+        acc[0] += dx * dx + dy * dy + dz * dz;
+        acc[1] += (double)ux * ux + (double)uy * uy + (double)uz * uz;
+        acc[2] += 1.0;
+        prev[id] = ux;
+        prev[stride + id] = uy;
+        prev[2 * stride + id] = uz;
+    }
+    block_store(acc, partials);
+}
+
+__global__ __launch_bounds__(64) void k_finalize(const double* partials, int nblocks,
+                                                 double* out)
+{
+    if (threadIdx.x < kRedVals) {
+        double t = 0.0;
+        for (int b = 0; b < nblocks; b++)
+            t += partials[b * kRedVals + threadIdx.x];
+        out[threadIdx.x] = t;
+    }
+}
+
+template <class P>
+static hipError_t reduce_box_t(const lbm_solver* s, const Box& b, double* partials,
+                               double* out, hipStream_t st)
+{
+    RedArgs a;
+    a.stride = s->stride;
+    a.nx = s->size_h[0];
+    a.nxy = s->size_h[0] * s->size_h[1];
+    a.ox = b.ox;
+    a.oy = b.oy;
+    a.oz = b.oz;
+    a.cells = b.cells();
+    a.div_bx = FastDiv((uint32_t)b.bx);
+    a.div_by = FastDiv((uint32_t)b.by);
+    hipLaunchKernelGGL(k_reduce_box<P>, dim3(kReduceBlocks), dim3(kRedBlock), 0, st,
+                       (const typename P::S*)s->dd, a, partials);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, st, partials, kReduceBlocks, out);
+    return hipGetLastError();
+}
+
+template <class P>
+static hipError_t residual_t(const lbm_solver* s, int layout, double* partials, double* out,
+                             hipStream_t st)
+{
+    hipLaunchKernelGGL(k_residual<P>, dim3(kReduceBlocks), dim3(kRedBlock), 0, st,
+                       (const typename P::S*)s->dd, s->flags,
+                       (typename P::C*)s->residual_prev, s->stride, (uint32_t)s->n, layout,
+                       partials);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, st, partials, kReduceBlocks, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_box(const lbm_solver* s, const Box& b, double* partials,
+                             double* out_dev, hipStream_t st)
+{
+    switch (s->p.precision) {
+    case LBM_F32: return reduce_box_t<StoreF32>(s, b, partials, out_dev, st);
+    case LBM_F64: return reduce_box_t<StoreF64>(s, b, partials, out_dev, st);
+    case LBM_F16S: return reduce_box_t<StoreF16S>(s, b, partials, out_dev, st);
+    case LBM_BF16S: return reduce_box_t<StoreBF16S>(s, b, partials, out_dev, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_residual(const lbm_solver* s, int layout, double* partials, double* out_dev,
+                           hipStream_t st)
+{
+    switch (s->p.precision) {
+    case LBM_F32: return residual_t<StoreF32>(s, layout, partials, out_dev, st);
+    case LBM_F64: return residual_t<StoreF64>(s, layout, partials, out_dev, st);
+    case LBM_F16S: return residual_t<StoreF16S>(s, layout, partials, out_dev, st);
+    case LBM_BF16S: return residual_t<StoreBF16S>(s, layout, partials, out_dev, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace lbm

@@ -1,0 +1,517 @@
+// Fused collide-stream A-A kernels (init, alpha, beta) for gfx950.
+//
+// Layout: structure of arrays, plane l of the populations at
+// dd + l * stride, cell id = (z * ny_h + y) * nx_h + x (x fastest), one byte
+// of flag per cell. One thread owns one cell; a 64-lane wavefront covers 64
+// consecutive cells, so every population plane is read and written as
+// contiguous 256-byte runs (4-byte elements).
+//
+// Alpha (even step, CLbmAlphaCPU.cpp:40-502): the cell reads its own 19 slots
+// and writes the post-collision value of direction l into slot opp(l).
+// Beta (odd step, CLbmBetaCPU.cpp:52-459): the cell reads plane l at
+// id + c_l (direction opp(l) arriving) and, after collision, writes direction
+// l into the same address. So in both kernels a thread reads and writes the
+// same 19 addresses and no two threads share an address: the step is
+// race-free in place without any barrier.
+//
+// Both steps are pure HBM streams: 19 loads + 19 stores per cell
+// (152 B/cell in FP32), ~1.6 flop/B, no matrix-core work.
+#include "lbm_d3q19.hpp"
+#include "lbm_internal.hpp"
+#include "lbm_storage.hpp"
+
+namespace lbm {
+
+constexpr int kBlock = 256;
+
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2). Renumber them so every XCD works on one contiguous range of cells:
+// neighbouring cells then share partial cache lines inside one L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb)
+{
+    const uint32_t q = nb >> 3, r = nb & 7u, x = b & 7u, k = b >> 3;
+    return x * q + (x < r ? x : r) + k;
+}
+
+template <class C>
+__device__ __forceinline__ uint32_t box_cell(const StepArgs<C>& a, uint32_t gid)
+{
+    const uint32_t t = a.div_bx.div(gid);
+    const uint32_t x = gid - t * a.div_bx.d;
+    const uint32_t z = a.div_by.div(t);
+    const uint32_t y = t - z * a.div_by.d;
+    return (uint32_t)((a.oz + (int32_t)z) * a.nxy + (a.oy + (int32_t)y) * a.nx + a.ox + (int32_t)x);
+}
+
+// ------------------------------------------------------------------ init ---
+
+// lbm_init.cu:41-208 / CLbmInitCPU.cpp:37-171: flags from the six face
+// conditions in x-fastest order (the first OBSTACLE wins, GHOST_LAYER can be
+// overwritten), populations at the rest equilibrium computed with the
+// reference's equilibrium arithmetic, u = 0, rho = 1.
+template <class P>
+__global__ __launch_bounds__(kBlock) void k_init(StepArgs<typename P::C> a, int ny,
+                                                 int nz, int4 bc_lo, int2 bc_hi)
+{
+    using C = typename P::C;
+    using S = typename P::S;
+    const uint32_t id = blockIdx.x * kBlock + threadIdx.x;
+    if (id >= (uint32_t)a.n)
+        return;
+    const int x = (int)(id % (uint32_t)a.nx);
+    const int y = (int)((id / (uint32_t)a.nx) % (uint32_t)ny);
+    const int z = (int)(id / (uint32_t)a.nxy);
+    const int bc[6] = {bc_lo.x, bc_lo.y, bc_lo.z, bc_lo.w, bc_hi.x, bc_hi.y};
+    int flag = kFluid;
+    if (x == 0)
+        flag = bc[0];
+    if (x == a.nx - 1 && flag != kObstacle)
+        flag = bc[1];
+    if (y == 0 && flag != kObstacle)
+        flag = bc[2];
+    if (y == ny - 1 && flag != kObstacle)
+        flag = bc[3];
+    if (z == 0 && flag != kObstacle)
+        flag = bc[4];
+    if (z == nz - 1 && flag != kObstacle)
+        flag = bc[5];
+    const_cast<uint8_t*>(a.flags)[id] = (uint8_t)flag;
+
+    S* dd = (S*)a.dd;
+    const C ux = (C)0, uy = (C)0, uz = (C)0, rho = (C)1;
+    C vela2 = ux * ux;
+    const C dd_param = rho - ((C)3 / (C)2) * (vela2);
+    C f[kQ];
+    f[0] = eq_a0(ux, vela2, dd_param);
+    f[1] = eq_a1(ux, vela2, dd_param);
+    vela2 = uy * uy;
+    f[2] = eq_a0(uy, vela2, dd_param);
+    f[3] = eq_a1(uy, vela2, dd_param);
+    C vab = ux + uy, vab2 = vab * vab;
+    f[4] = eq_4(vab, vab2, dd_param);
+    f[5] = eq_5(vab, vab2, dd_param);
+    vab = ux - uy; vab2 = vab * vab;
+    f[6] = eq_4(vab, vab2, dd_param);
+    f[7] = eq_5(vab, vab2, dd_param);
+    vab = ux + uz; vab2 = vab * vab;
+    f[8] = eq_4(vab, vab2, dd_param);
+    f[9] = eq_5(vab, vab2, dd_param);
+    vab = ux - uz; vab2 = vab * vab;
+    f[10] = eq_4(vab, vab2, dd_param);
+    f[11] = eq_5(vab, vab2, dd_param);
+    vab = uy + uz; vab2 = vab * vab;
+    f[12] = eq_4(vab, vab2, dd_param);
+    f[13] = eq_5(vab, vab2, dd_param);
+    vab = uy - uz; vab2 = vab * vab;
+    f[14] = eq_4(vab, vab2, dd_param);
+    f[15] = eq_5(vab, vab2, dd_param);
+    vela2 = uz * uz;
+    f[16] = eq_a0(uz, vela2, dd_param);
+    f[17] = eq_a1(uz, vela2, dd_param);
+    f[18] = eq_18(dd_param);
+#pragma unroll
+    for (int l = 0; l < kQ; l++)
+        dd[(int64_t)l * a.stride + id] = P::store(f[l], l);
+    if (a.store_u) {
+        a.u[id] = ux;
+        a.u[a.stride + id] = uy;
+        a.u[2 * a.stride + id] = uz;
+    }
+    if (a.store_rho)
+        a.rho[id] = rho;
+}
+
+// ----------------------------------------------------------------- alpha ---
+
+// One cell of CLbmAlphaCPU::alphaKernelCPU (CLbmAlphaCPU.cpp:75-494). The
+// reference's `tmp` is a #define for `rho` (:199): every gravity term
+// overwrites rho and the stored density is the last gravity term.
+template <class P>
+__device__ __forceinline__ void alpha_cell(const StepArgs<typename P::C>& a, uint32_t id)
+{
+    using C = typename P::C;
+    using S = typename P::S;
+    S* __restrict__ dd = (S*)a.dd;
+    const int flag = a.flags[id];
+    if (flag == kGhostLayer)
+        return;
+
+    C d[kQ];
+#pragma unroll
+    for (int l = 0; l < kQ; l++)
+        d[l] = P::load(dd[(int64_t)l * a.stride + id], l);
+    C rho, ux, uy, uz;
+    moments(d, rho, ux, uy, uz);
+
+    if (flag == kFluid || flag == kVelocityInjection) {
+        const C inv_tau = a.inv_tau;
+        const C g0 = a.g0, g1 = a.g1, g2 = a.g2;
+        const bool fluid = (flag == kFluid);
+        C vel2, vela2, vab, dd_param;
+        C e[kQ];
+        if (!fluid) {
+            ux = a.lid;
+            uy = (C)0;
+            uz = (C)0;
+            rho = (C)1;
+        }
+        vel2 = ux * ux + uy * uy + uz * uz;
+        dd_param = rho - ((C)3 / (C)2) * vel2;
+        // post-collision value of direction l: relaxation for FLUID (d += inv_tau*(eq - d)),
+        // the equilibrium itself for VELOCITY_INJECTION
+#define LBM_RELAX(l, eqv) e[l] = fluid ? (d[l] + inv_tau * ((eqv) - d[l])) : (eqv)
+        rho = g0 * ((C)1 / (C)18) * rho;
+        vela2 = ux * ux;
+        LBM_RELAX(1, eq_a1(ux, vela2, dd_param));
+        e[1] -= rho;
+        LBM_RELAX(0, eq_a0(ux, vela2, dd_param));
+        e[0] += rho;
+
+        rho = g1 * ((C)-1 / (C)18) * rho;
+        vela2 = uy * uy;
+        LBM_RELAX(3, eq_a1(uy, vela2, dd_param));
+        e[3] -= rho;
+        LBM_RELAX(2, eq_a0(uy, vela2, dd_param));
+        e[2] += rho;
+
+        vab = ux + uy;
+        vela2 = vab * vab;
+        rho = (g0 - g1) * ((C)1 / (C)36) * rho;
+        LBM_RELAX(5, eq_5(vab, vela2, dd_param));
+        e[5] -= rho;
+        LBM_RELAX(4, eq_4(vab, vela2, dd_param));
+        e[4] += rho;
+
+        vab = ux - uy;
+        vela2 = vab * vab;
+        rho = (g0 + g1) * ((C)1 / (C)36) * rho;
+        LBM_RELAX(7, eq_5(vab, vela2, dd_param));
+        e[7] -= rho;
+        LBM_RELAX(6, eq_4(vab, vela2, dd_param));
+        e[6] += rho;
+
+        vab = ux + uz;
+        vela2 = vab * vab;
+        rho = (g0 + g2) * ((C)1 / (C)36) * rho;
+        LBM_RELAX(9, eq_5(vab, vela2, dd_param));
+        e[9] -= rho;
+        LBM_RELAX(8, eq_4(vab, vela2, dd_param));
+        e[8] += rho;
+
+        rho = (g0 - g2) * ((C)1 / (C)36) * rho;
+        vab = ux - uz;
+        vela2 = vab * vab;
+        LBM_RELAX(11, eq_5(vab, vela2, dd_param));
+        e[11] -= rho;
+        LBM_RELAX(10, eq_4(vab, vela2, dd_param));
+        e[10] += rho;
+
+        vab = uy + uz;
+        vela2 = vab * vab;
+        rho = (g2 - g1) * ((C)1 / (C)36) * rho;
+        LBM_RELAX(13, eq_5(vab, vela2, dd_param));
+        e[13] -= rho;
+        LBM_RELAX(12, eq_4(vab, vela2, dd_param));
+        e[12] += rho;
+
+        vab = uy - uz;
+        vela2 = vab * vab;
+        rho = (g2 + g1) * ((C)-1 / (C)36) * rho;
+        LBM_RELAX(15, eq_5(vab, vela2, dd_param));
+        e[15] -= rho;
+        LBM_RELAX(14, eq_4(vab, vela2, dd_param));
+        e[14] += rho;
+
+        vela2 = uz * uz;
+        rho = g2 * ((C)1 / (C)18) * rho;
+        LBM_RELAX(17, eq_a1(uz, vela2, dd_param));
+        e[17] -= rho;
+        LBM_RELAX(16, eq_a0(uz, vela2, dd_param));
+        e[16] += rho;
+
+        LBM_RELAX(18, eq_18(dd_param));
+#undef LBM_RELAX
+        // direction l goes to slot opp(l) of this cell
+#pragma unroll
+        for (int l = 0; l < kQ; l++)
+            dd[(int64_t)opp(l) * a.stride + id] = P::store(e[l], opp(l));
+    } else if (flag == kObstacle) {
+        // no stores: the A-A layout makes bounce-back implicit (:330-343)
+        if (a.store_u) {
+            ux = (C)0;
+            uy = (C)0;
+            uz = (C)0;
+        }
+    }
+
+    if (a.store_u) {
+        a.u[id] = ux;
+        a.u[a.stride + id] = uy;
+        a.u[2 * a.stride + id] = uz;
+    }
+    if (a.store_rho)
+        a.rho[id] = rho;
+}
+
+template <class P, bool FULL>
+__global__ __launch_bounds__(kBlock) void k_alpha(StepArgs<typename P::C> a)
+{
+    const uint32_t gid = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (gid >= a.cells)
+        return;
+    alpha_cell<P>(a, FULL ? gid : box_cell(a, gid));
+}
+
+// ------------------------------------------------------------------ beta ---
+
+// One cell of CLbmBetaCPU::betaKernelCPU (CLbmBetaCPU.cpp:83-451).
+// `dd_param` is a #define for `rho` (:224): the stored FLUID density is
+// rho - 1.5 |j|^2. OBSTACLE swaps opposite pairs, which writes back exactly
+// what was read, so its stores are skipped. GHOST_LAYER passes values through.
+// Neighbour indices wrap modulo the cell count (domainWrap, :44-50).
+template <class P, bool WRAP>
+__device__ __forceinline__ void beta_cell(const StepArgs<typename P::C>& a, uint32_t id)
+{
+    using C = typename P::C;
+    using S = typename P::S;
+    S* __restrict__ dd = (S*)a.dd;
+    const int flag = a.flags[id];
+
+    int32_t pos[kQ];
+#pragma unroll
+    for (int l = 0; l < kQ; l++) {
+        int32_t j = (int32_t)id + (cx(l) + cy(l) * a.nx + cz(l) * a.nxy);
+        if (WRAP) {
+            j = (j >= a.n) ? j - a.n : j;
+            j = (j < 0) ? j + a.n : j;
+        }
+        pos[l] = j;
+    }
+    // plane l at id + c_l holds direction opp(l) arriving at this cell
+    C d[kQ];
+#pragma unroll
+    for (int l = 0; l < kQ; l++)
+        d[opp(l)] = P::load(dd[(int64_t)l * a.stride + pos[l]], l);
+
+    C rho, ux, uy, uz;
+    moments(d, rho, ux, uy, uz);
+
+    if (flag == kFluid) {
+        const C inv_tau = a.inv_tau;
+        C vel2 = ux * ux + uy * uy + uz * uz;
+        rho = rho - ((C)3 / (C)2) * (vel2);
+        C vela2 = ux * ux;
+        d[0] += inv_tau * (eq_a0(ux, vela2, rho) - d[0]);
+        d[1] += inv_tau * (eq_a1(ux, vela2, rho) - d[1]);
+        vela2 = uy * uy;
+        d[2] += inv_tau * (eq_a0(uy, vela2, rho) - d[2]);
+        d[3] += inv_tau * (eq_a1(uy, vela2, rho) - d[3]);
+        vel2 = ux + uy;
+        vela2 = vel2 * vel2;
+        d[4] += inv_tau * (eq_4(vel2, vela2, rho) - d[4]);
+        d[5] += inv_tau * (eq_5(vel2, vela2, rho) - d[5]);
+        vel2 = ux - uy;
+        vela2 = vel2 * vel2;
+        d[6] += inv_tau * (eq_4(vel2, vela2, rho) - d[6]);
+        d[7] += inv_tau * (eq_5(vel2, vela2, rho) - d[7]);
+        vel2 = ux + uz;
+        vela2 = vel2 * vel2;
+        d[8] += inv_tau * (eq_4(vel2, vela2, rho) - d[8]);
+        d[9] += inv_tau * (eq_5(vel2, vela2, rho) - d[9]);
+        vel2 = ux - uz;
+        vela2 = vel2 * vel2;
+        d[10] += inv_tau * (eq_4(vel2, vela2, rho) - d[10]);
+        d[11] += inv_tau * (eq_5(vel2, vela2, rho) - d[11]);
+        vel2 = uy + uz;
+        vela2 = vel2 * vel2;
+        d[12] += inv_tau * (eq_4(vel2, vela2, rho) - d[12]);
+        d[13] += inv_tau * (eq_5(vel2, vela2, rho) - d[13]);
+        vel2 = uy - uz;
+        vela2 = vel2 * vel2;
+        d[14] += inv_tau * (eq_4(vel2, vela2, rho) - d[14]);
+        d[15] += inv_tau * (eq_5(vel2, vela2, rho) - d[15]);
+        vela2 = uz * uz;
+        d[16] += inv_tau * (eq_a0(uz, vela2, rho) - d[16]);
+        d[17] += inv_tau * (eq_a1(uz, vela2, rho) - d[17]);
+        d[18] += inv_tau * (eq_18(rho) - d[18]);
+    } else if (flag == kVelocityInjection) {
+        ux = a.lid;
+        uy = (C)0;
+        uz = (C)0;
+        rho = (C)1;
+        C vel2 = ux * ux + uy * uy + uz * uz;
+        rho = rho - ((C)3 / (C)2) * vel2;
+        C vela2 = ux * ux;
+        d[0] = eq_a0(ux, vela2, rho);
+        d[1] = eq_a1(ux, vela2, rho);
+        vela2 = uy * uy;
+        d[2] = eq_a0(uy, vela2, rho);
+        d[3] = eq_a1(uy, vela2, rho);
+        vel2 = ux + uy;
+        vela2 = vel2 * vel2;
+        d[4] = eq_4(vel2, vela2, rho);
+        d[5] = eq_5(vel2, vela2, rho);
+        vel2 = ux - uy;
+        vela2 = vel2 * vel2;
+        d[6] = eq_4(vel2, vela2, rho);
+        d[7] = eq_5(vel2, vela2, rho);
+        vel2 = ux + uz;
+        vela2 = vel2 * vel2;
+        d[8] = eq_4(vel2, vela2, rho);
+        d[9] = eq_5(vel2, vela2, rho);
+        vel2 = ux - uz;
+        vela2 = vel2 * vel2;
+        d[10] = eq_4(vel2, vela2, rho);
+        d[11] = eq_5(vel2, vela2, rho);
+        vel2 = uy + uz;
+        vela2 = vel2 * vel2;
+        d[12] = eq_4(vel2, vela2, rho);
+        d[13] = eq_5(vel2, vela2, rho);
+        vel2 = uy - uz;
+        vela2 = vel2 * vel2;
+        d[14] = eq_4(vel2, vela2, rho);
+        d[15] = eq_5(vel2, vela2, rho);
+        vela2 = uz * uz;
+        d[16] = eq_a0(uz, vela2, rho);
+        d[17] = eq_a1(uz, vela2, rho);
+        d[18] = eq_18(rho);
+    } else if (flag == kObstacle) {
+        if (a.store_u) {
+            ux = (C)0;
+            uy = (C)0;
+            uz = (C)0;
+        }
+    }
+
+    if (flag != kObstacle) {
+        // direction l goes to slot l of neighbour id + c_l
+#pragma unroll
+        for (int l = 0; l < kQ; l++)
+            dd[(int64_t)l * a.stride + pos[l]] = P::store(d[l], l);
+    }
+
+    if (a.store_u) {
+        a.u[id] = ux;
+        a.u[a.stride + id] = uy;
+        a.u[2 * a.stride + id] = uz;
+    }
+    if (a.store_rho)
+        a.rho[id] = rho;
+}
+
+template <class P, bool FULL>
+__global__ __launch_bounds__(kBlock) void k_beta(StepArgs<typename P::C> a)
+{
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t gid = blk * kBlock + threadIdx.x;
+    // One decision per workgroup: neighbours of every cell of this block stay
+    // inside [0, n) unless the block touches the first or last slab.
+    const uint32_t g_first = blk * kBlock;
+    const uint32_t g_last = min(g_first + kBlock - 1, a.cells - 1);
+    const uint32_t id_first = FULL ? g_first : box_cell(a, g_first);
+    const uint32_t id_last = FULL ? g_last : box_cell(a, g_last);
+    const int64_t reach = (int64_t)a.nxy + a.nx + 1;
+    const bool wrap = (int64_t)id_first < reach || (int64_t)id_last + reach >= (int64_t)a.n;
+    if (gid >= a.cells)
+        return;
+    const uint32_t id = FULL ? gid : box_cell(a, gid);
+    if (wrap)
+        beta_cell<P, true>(a, id);
+    else
+        beta_cell<P, false>(a, id);
+}
+
+// -------------------------------------------------------------- launches ---
+
+template <class P>
+static StepArgs<typename P::C> make_args(const lbm_solver* s, const Box& b)
+{
+    using C = typename P::C;
+    StepArgs<C> a;
+    a.dd = s->dd;
+    a.flags = s->flags;
+    a.rho = (C*)s->rho;
+    a.u = (C*)s->u;
+    a.stride = s->stride;
+    a.n = (int32_t)s->n;
+    a.nx = s->size_h[0];
+    a.nxy = s->size_h[0] * s->size_h[1];
+    a.ox = b.ox;
+    a.oy = b.oy;
+    a.oz = b.oz;
+    a.cells = b.cells();
+    a.div_bx = FastDiv((uint32_t)b.bx);
+    a.div_by = FastDiv((uint32_t)b.by);
+    a.inv_tau = (C)s->p.inv_tau;
+    a.lid = (C)s->p.lid_velocity;
+    a.g0 = (C)s->p.gravitation[0];
+    a.g1 = (C)s->p.gravitation[1];
+    a.g2 = (C)s->p.gravitation[2];
+    a.store_rho = s->p.store_densities;
+    a.store_u = s->p.store_velocities;
+    return a;
+}
+
+static bool is_full(const lbm_solver* s, const Box& b)
+{
+    return b.ox == 0 && b.oy == 0 && b.oz == 0 && b.bx == s->size_h[0] &&
+           b.by == s->size_h[1] && b.bz == s->size_h[2];
+}
+
+template <class P>
+static hipError_t init_t(const lbm_solver* s, hipStream_t st)
+{
+    Box all{0, 0, 0, s->size_h[0], s->size_h[1], s->size_h[2]};
+    auto a = make_args<P>(s, all);
+    const uint32_t blocks = (uint32_t)((s->n + kBlock - 1) / kBlock);
+    int4 lo = make_int4(s->p.bc[0], s->p.bc[1], s->p.bc[2], s->p.bc[3]);
+    int2 hi = make_int2(s->p.bc[4], s->p.bc[5]);
+    hipLaunchKernelGGL(k_init<P>, dim3(blocks), dim3(kBlock), 0, st, a, s->size_h[1],
+                       s->size_h[2], lo, hi);
+    return hipGetLastError();
+}
+
+template <class P>
+static hipError_t alpha_t(const lbm_solver* s, const Box& b, hipStream_t st)
+{
+    auto a = make_args<P>(s, b);
+    const uint32_t blocks = (a.cells + kBlock - 1) / kBlock;
+    if (is_full(s, b))
+        hipLaunchKernelGGL((k_alpha<P, true>), dim3(blocks), dim3(kBlock), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_alpha<P, false>), dim3(blocks), dim3(kBlock), 0, st, a);
+    return hipGetLastError();
+}
+
+template <class P>
+static hipError_t beta_t(const lbm_solver* s, const Box& b, hipStream_t st)
+{
+    auto a = make_args<P>(s, b);
+    const uint32_t blocks = (a.cells + kBlock - 1) / kBlock;
+    if (is_full(s, b))
+        hipLaunchKernelGGL((k_beta<P, true>), dim3(blocks), dim3(kBlock), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_beta<P, false>), dim3(blocks), dim3(kBlock), 0, st, a);
+    return hipGetLastError();
+}
+
+#define LBM_DISPATCH(fn, ...)                                  \
+    switch (s->p.precision) {                                  \
+    case LBM_F32: return fn<StoreF32>(__VA_ARGS__);            \
+    case LBM_F64: return fn<StoreF64>(__VA_ARGS__);            \
+    case LBM_F16S: return fn<StoreF16S>(__VA_ARGS__);          \
+    case LBM_BF16S: return fn<StoreBF16S>(__VA_ARGS__);        \
+    default: return hipErrorInvalidValue;                      \
+    }
+
+hipError_t launch_init(const lbm_solver* s, hipStream_t st) { LBM_DISPATCH(init_t, s, st) }
+hipError_t launch_alpha(const lbm_solver* s, const Box& b, hipStream_t st)
+{
+    LBM_DISPATCH(alpha_t, s, b, st)
+}
+hipError_t launch_beta(const lbm_solver* s, const Box& b, hipStream_t st)
+{
+    LBM_DISPATCH(beta_t, s, b, st)
+}
+
+}  // namespace lbm

@@ -1,0 +1,269 @@
+/*
+ * C ABI of the MI355X-native D3Q19 alpha/beta A-A solver (libcavityflow_hip.so).
+ *
+ * This is the drop-in boundary for the hot path of saleemayman/cavityflow_cuda:
+ * the CLbmSolver<T> operator surface (CLbmSolver.hpp:68-88, CLbmSolverGPU.cuh:56-97)
+ * driven by CController / CManager. Each entry point below names the reference
+ * method it replaces. Plain pointers and sizes only.
+ *
+ * Conventions (the reference's, CLbmSolverCPU.cpp:142-244, CLbmSolverGPU.cu:516-1328):
+ *  - A solver owns one subdomain of size[3] interior cells plus a one-cell halo
+ *    on each face: size_h = size + 2 per axis.
+ *  - Boxes (origin[3], box[3]) are in halo coordinates, x fastest, and must
+ *    satisfy origin >= 0, box > 0, origin + box <= size_h. A NULL origin means
+ *    the reference's whole-domain overload: the full halo-inclusive domain for
+ *    the steps (CLbmSolverCPU.cpp:270-281), the interior origin (1,1,1),
+ *    box = size for the copies (CLbmSolverCPU.cpp:552-559).
+ *  - Copy buffers are dense [component][z][y][x] over the box: 19 components
+ *    for populations, 3 for velocities, 1 for densities and flags.
+ *  - Population/velocity/density buffers hold `double` for LBM_F64 and `float`
+ *    for every other precision (shifted 16-bit storage is converted).
+ *  - Errors are returned as negative lbm_status codes; nothing calls exit().
+ *    lbm_last_error() holds a message for the calling thread.
+ *  - Streams: every call is ordered on `stream` (NULL = the default stream).
+ *    Calls with host buffers are synchronous; calls with device buffers and
+ *    the steps are asynchronous, as the reference GPU solver's are.
+ *  - One solver is used from one host thread at a time.
+ */
+#ifndef CAVITYFLOW_LBM_ABI_H
+#define CAVITYFLOW_LBM_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* HIP's own opaque stream handle type (hip_runtime_api.h). */
+typedef struct ihipStream_t* hipStream_t;
+
+typedef struct lbm_solver lbm_solver;
+
+/* enum Flag, common.h:54-61 */
+enum lbm_flag {
+    LBM_OBSTACLE = 1,
+    LBM_FLUID = 2,
+    LBM_VELOCITY_INJECTION = 4,
+    LBM_GHOST_LAYER = 8
+};
+
+/* enum Direction, common.h:63-71 */
+enum lbm_direction {
+    LBM_LEFT = 0,
+    LBM_RIGHT = 1,
+    LBM_BOTTOM = 2,
+    LBM_TOP = 3,
+    LBM_BACK = 4,
+    LBM_FRONT = 5
+};
+
+/* Population storage. F32 and F64 reproduce the reference bit for bit;
+ * F16S / BF16S store f_l - w_l in 16 bits with FP32 collision arithmetic. */
+enum lbm_precision {
+    LBM_F32 = 0,
+    LBM_F64 = 1,
+    LBM_F16S = 2,
+    LBM_BF16S = 3
+};
+
+enum lbm_status {
+    LBM_OK = 0,
+    LBM_ERR_ARG = -1,         /* bad argument (NULL, bad enum, bad flag value) */
+    LBM_ERR_BOX = -2,         /* box outside the halo-inclusive domain (reference: assert) */
+    LBM_ERR_HIP = -3,         /* a HIP runtime call failed */
+    LBM_ERR_NOMEM = -4,       /* device allocation failed */
+    LBM_ERR_PARAMS = -5,      /* no valid time step (reference: exit, CLbmSolver.cpp:263-272) */
+    LBM_ERR_STATE = -6,       /* operation not available in this solver state */
+    LBM_ERR_UNSUPPORTED = -7  /* e.g. outputs requested but not stored */
+};
+
+/* Construction parameters of one subdomain solver: the quantities the
+ * CLbmSolver constructor derives (CLbmSolver.cpp:28-281) and the
+ * CLbmSolverGPU constructor consumes (CLbmSolverGPU.cu:31-161). */
+typedef struct lbm_params {
+    int size[3];            /* CDomain::getSize(), interior cells per axis (>= 1) */
+    int bc[6];              /* boundaryConditions: LEFT, RIGHT, BOTTOM, TOP, BACK, FRONT */
+    int precision;          /* enum lbm_precision */
+    double inv_tau;         /* CLbmSolver::tauInv, rounded to the arithmetic type */
+    double lid_velocity;    /* CLbmSolver::velocityDimLess[0] */
+    double gravitation[3];  /* CLbmSolver::accelerationDimLess */
+    int store_densities;    /* CLbmSolver::storeDensities */
+    int store_velocities;   /* CLbmSolver::storeVelocities */
+    int device;             /* HIP device ordinal (reference: id % device count) */
+} lbm_params;
+
+/* Input of the CLbmSolver constructor's time-step adaptation. */
+typedef struct lbm_physics {
+    double length_x;        /* CDomain::getLength()[0] of this solver's domain */
+    int size_x;             /* CDomain::getSize()[0] */
+    double length_max;      /* CConfiguration::domainLength.max() */
+    double velocity[3];     /* CConfiguration::velocity */
+    double acceleration[3]; /* CConfiguration::acceleration */
+    double viscosity;       /* <= 0 selects the default-Reynolds viscosity */
+    double timestep;
+    double max_velocity;     /* CConfiguration::maxVelocityDimLess */
+    double max_acceleration; /* CConfiguration::maxAccelerationDimLess */
+} lbm_physics;
+
+typedef struct lbm_derived {
+    double velocity_dimless[3];
+    double acceleration_dimless[3];
+    double viscosity;
+    double viscosity_dimless;
+    double tau;
+    double inv_tau;
+    double timestep;
+} lbm_derived;
+
+/* Device arrays of a solver, for callers that fuse their own kernels. */
+typedef struct lbm_device_arrays {
+    void* dd;            /* populations, plane l at dd + l * plane_stride elements */
+    uint8_t* flags;      /* one byte per cell */
+    void* densities;     /* NULL unless store_densities */
+    void* velocities;    /* component c at velocities + c * plane_stride; NULL unless stored */
+    int64_t cells;       /* N_h = size_h[0] * size_h[1] * size_h[2] */
+    int64_t plane_stride;/* elements between population planes (>= cells, 256-aligned) */
+    int size_h[3];
+    int precision;
+    int element_bytes;   /* bytes per stored population */
+} lbm_device_arrays;
+
+/* ---- lifecycle ---------------------------------------------------------- */
+
+/* CLbmSolver<T>::CLbmSolver time-step adaptation (CLbmSolver.cpp:75-281),
+ * in float for every precision except LBM_F64 (double). Returns LBM_OK or
+ * LBM_ERR_PARAMS where the reference exits. */
+int lbm_compute_parameters(const lbm_physics* in, int precision, lbm_derived* out);
+
+/* CLbmSolverGPU<T>::CLbmSolverGPU (CLbmSolverGPU.cu:31-161): selects the
+ * device, allocates the SoA arrays, runs the init kernel (lbm_init.cu:41-208:
+ * flags from the six face conditions, x fastest; populations at rest
+ * equilibrium; u = 0, rho = 1). Synchronous. */
+int lbm_create(const lbm_params* params, lbm_solver** out);
+
+/* CLbmSolverGPU<T>::~CLbmSolverGPU (CLbmSolverGPU.cu:163-172). */
+int lbm_destroy(lbm_solver* solver);
+
+int lbm_get_device_arrays(const lbm_solver* solver, lbm_device_arrays* out);
+
+/* ---- time steps --------------------------------------------------------- */
+
+/* simulationStepAlpha() / simulationStepAlpha(origin, size[, stream])
+ * (CLbmSolver.hpp:68-69, CLbmSolverGPU.cu:174-338; CPU arithmetic
+ * CLbmAlphaCPU.cpp:40-502). Even A-A step, in place. */
+int lbm_simulation_step_alpha(lbm_solver* solver, const int origin[3], const int size[3],
+                              hipStream_t stream);
+
+/* simulationStepBeta() / simulationStepBeta(origin, size[, stream])
+ * (CLbmSolver.hpp:70-71, CLbmSolverGPU.cu:340-514; CPU arithmetic
+ * CLbmBetaCPU.cpp:52-459). Odd A-A step, in place, linear-index wrap. */
+int lbm_simulation_step_beta(lbm_solver* solver, const int origin[3], const int size[3],
+                             hipStream_t stream);
+
+/* ---- box copies (halo exchange, checkpoint, extraction) ------------------ */
+
+/* getDensityDistributions(origin, size, dst[, stream]) / (dst)
+ * (CLbmSolver.hpp:73-74, CLbmSolverGPU.cu:516-611, CLbmSolverCPU.cpp:497-559). */
+int lbm_get_density_distributions(lbm_solver* solver, const int origin[3], const int size[3],
+                                  void* dst, int dst_on_device, hipStream_t stream);
+
+/* setDensityDistributions(origin, size, src[, stream]) / (src)
+ * (CLbmSolver.hpp:75-76, CLbmSolverGPU.cu:723-820, CLbmSolverCPU.cpp:643-706). */
+int lbm_set_density_distributions(lbm_solver* solver, const int origin[3], const int size[3],
+                                  const void* src, int src_on_device, hipStream_t stream);
+
+/* setDensityDistributions(origin, size, direction, src[, stream])
+ * (CLbmSolverCPU.hpp:80, CLbmSolverGPU.cu:613-721, CLbmSolverCPU.cpp:562-641):
+ * copies only the lattice vectors l with norm(direction) . c_l > 0. */
+int lbm_set_density_distributions_dir(lbm_solver* solver, const int origin[3],
+                                      const int size[3], int direction, const void* src,
+                                      int src_on_device, hipStream_t stream);
+
+/* getFlags / setFlags (CLbmSolver.hpp:77-80, CLbmSolverGPU.cu:822-986). Host
+ * buffers of int32 Flag values; setFlags accepts only the four Flag values. */
+int lbm_get_flags(lbm_solver* solver, const int origin[3], const int size[3], int32_t* dst);
+int lbm_set_flags(lbm_solver* solver, const int origin[3], const int size[3],
+                  const int32_t* src);
+
+/* getVelocities / setVelocities (CLbmSolver.hpp:81-84, CLbmSolverGPU.cu:988-1160,
+ * CLbmSolverCPU.cpp:833-960). LBM_ERR_UNSUPPORTED unless store_velocities. */
+int lbm_get_velocities(lbm_solver* solver, const int origin[3], const int size[3], void* dst,
+                       int dst_on_device, hipStream_t stream);
+int lbm_set_velocities(lbm_solver* solver, const int origin[3], const int size[3],
+                       const void* src, int src_on_device, hipStream_t stream);
+
+/* getDensities / setDensities (CLbmSolver.hpp:85-88, CLbmSolverGPU.cu:1162-1328,
+ * CLbmSolverCPU.cpp:962-1084). LBM_ERR_UNSUPPORTED unless store_densities. */
+int lbm_get_densities(lbm_solver* solver, const int origin[3], const int size[3], void* dst,
+                      int dst_on_device, hipStream_t stream);
+int lbm_set_densities(lbm_solver* solver, const int origin[3], const int size[3],
+                      const void* src, int src_on_device, hipStream_t stream);
+
+/* ---- halo exchange faces (CComm shapes, CManager.cpp:135-202) ------------ */
+
+/* Face boxes of the reference's CComm descriptors for face `direction`:
+ * which = 0 -> sendOrigin/sendSize (plane at offset 1 from the face),
+ * which = 1 -> recvOrigin/recvSize (the halo plane). Both span the full
+ * halo-inclusive extent of the other two axes. */
+int lbm_face_box(const lbm_solver* solver, int direction, int which, int origin[3],
+                 int size[3]);
+
+/* Elements (19 per cell) of one face buffer for `direction`. */
+int64_t lbm_face_elements(const lbm_solver* solver, int direction);
+
+/* Pack one face into a dense device buffer [l][z][y][x], the device-side
+ * half of the exchange in CController::stepAlpha / stepBeta
+ * (CController.cpp:369-570, 807-1008):
+ *   phase 0 (after alpha): the send plane (offset 1), all 19 populations;
+ *   phase 1 (after beta) : the halo plane, all 19 populations. */
+int lbm_pack_face(lbm_solver* solver, int direction, int phase, void* dev_buf,
+                  hipStream_t stream);
+
+/* Unpack a buffer packed by the neighbour across face `direction`:
+ *   phase 0: into the halo plane, all 19 populations (CController.cpp:411-435);
+ *   phase 1: into the send plane, only the populations pointing into the
+ *            domain (setDensityDistributions with direction, :849-874). */
+int lbm_unpack_face(lbm_solver* solver, int direction, int phase, const void* dev_buf,
+                    hipStream_t stream);
+
+/* Single-GPU stand-in for the neighbour send/recv of one face: packs `from`'s
+ * face `direction` (phase as in lbm_pack_face) through `from`'s device
+ * staging buffer and unpacks it into `to` across the opposite face, e.g.
+ * from.RIGHT -> to.LEFT. Both solvers must live on the same device and have
+ * the same halo-inclusive extent on the two axes of the face. */
+int lbm_exchange_face(lbm_solver* from, int direction, int phase, lbm_solver* to,
+                      hipStream_t stream);
+
+/* Device memory for face buffers and other caller-owned scratch. */
+int lbm_device_alloc(int device, size_t bytes, void** out);
+int lbm_device_free(int device, void* ptr);
+
+/* ---- FP64 reductions (no reference counterpart; used by the steady-state
+ *      loop and the mass-conservation checks) ------------------------------ */
+
+/* Sum over the box of all 19 populations of every cell, in FP64:
+ * out[0] = sum f, out[1..3] = sum c_l f (momentum in the layout the
+ * populations are in), out[4] = number of cells. */
+int lbm_reduce_box(lbm_solver* solver, const int origin[3], const int size[3], double out[5],
+                   hipStream_t stream);
+
+/* Steady-state residual ||u - u_prev||_2 / ||u||_2 over FLUID cells, in FP64,
+ * where u is the momentum computed from the populations (`layout` 0: after
+ * beta, natural slots; 1: after alpha, swapped slots) and u_prev the value
+ * saved by the previous call on this solver. Saves u for the next call.
+ * The first call returns +inf. Synchronous. */
+int lbm_residual(lbm_solver* solver, int layout, double* residual, hipStream_t stream);
+
+/* ---- misc ---------------------------------------------------------------- */
+
+int lbm_sync(lbm_solver* solver, hipStream_t stream);
+const char* lbm_last_error(void);
+const char* lbm_status_string(int status);
+int lbm_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CAVITYFLOW_LBM_ABI_H */

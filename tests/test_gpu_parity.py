@@ -1,0 +1,271 @@
+"""GPU parity of the HIP A-A kernels against the oracle and the golden vectors.
+
+F32 and F64 are bit-exact by construction (same operation order, no FMA
+contraction), so every comparison here is on bytes. Each test prints its case
+before touching the GPU.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+OBST, FLUID, INJ, GHOST = 1, 2, 4, 8
+PREC = {np.float32: "f32", np.float64: "f64"}
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def full_box(s):
+    return (0, 0, 0), s.size_h
+
+
+def load_state(g_solver, o_solver):
+    """Copy the oracle solver's whole state into the GPU solver."""
+    o, z = full_box(g_solver)
+    g_solver.setDensityDistributions(o_solver.dd, o, z)
+    g_solver.setFlags(o_solver.flags, o, z)
+    g_solver.setVelocities(o_solver.u, o, z)
+    g_solver.setDensities(o_solver.rho, o, z)
+
+
+def same_state(g_solver, o_solver, what=""):
+    o, z = full_box(g_solver)
+    dd = g_solver.getDensityDistributions(o, z)
+    u = g_solver.getVelocities(o, z)
+    rho = g_solver.getDensities(o, z)
+    assert dd.tobytes() == o_solver.dd.tobytes(), f"populations differ {what}"
+    assert u.tobytes() == o_solver.u.tobytes(), f"velocities differ {what}"
+    assert rho.tobytes() == o_solver.rho.tobytes(), f"densities differ {what}"
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("size,bc", [((6, 5, 7), (OBST,) * 6),
+                                     ((9, 4, 6), (OBST, GHOST, OBST, OBST, GHOST, OBST)),
+                                     ((3, 3, 3), (GHOST,) * 6)])
+def test_init_matches_oracle(hip, oracle_cpu, dtype, size, bc):
+    say("init", dtype.__name__, size, bc)
+    g = hip.LbmSolverHIP(size, bc, PREC[dtype])
+    o = oracle_cpu.OracleSolver(size, bc, dtype=dtype)
+    same_state(g, o, "after init")
+    assert (g.getFlags(*full_box(g)) == o.flags).all()
+    g.close()
+
+
+@pytest.mark.parametrize("dtype,name", [(np.float32, "step_f32.npz"), (np.float64, "step_f64.npz")])
+def test_steps_match_reference_golden(hip, dtype, name):
+    g = np.load(os.path.join(GOLDEN, name))
+    size = tuple(int(v) for v in g["size"])
+    say("golden step", dtype.__name__, size)
+    s = hip.LbmSolverHIP(size, (OBST,) * 6, PREC[dtype], float(g["inv_tau"]), float(g["lid"]),
+                         tuple(float(v) for v in g["g"]))
+    o, z = full_box(s)
+    s.setDensityDistributions(g["dd0"], o, z)
+    s.setFlags(g["flags"], o, z)
+    s.setVelocities(np.zeros(3 * int(np.prod(z)), dtype), o, z)
+    s.setDensities(np.zeros(int(np.prod(z)), dtype), o, z)
+    for phase, step in (("alpha", s.simulationStepAlpha), ("beta", s.simulationStepBeta)):
+        step()
+        assert s.getDensityDistributions(o, z).tobytes() == g[f"{phase}_dd"].tobytes(), phase
+        assert s.getVelocities(o, z).tobytes() == g[f"{phase}_u"].tobytes(), phase
+        assert s.getDensities(o, z).tobytes() == g[f"{phase}_rho"].tobytes(), phase
+    bo, bz = tuple(int(v) for v in g["box_origin"]), tuple(int(v) for v in g["box_size"])
+    s.simulationStepAlpha(bo, bz)
+    s.simulationStepBeta(bo, bz)
+    assert s.getDensityDistributions(o, z).tobytes() == g["box_dd"].tobytes()
+    assert s.getVelocities(o, z).tobytes() == g["box_u"].tobytes()
+    assert s.getDensities(o, z).tobytes() == g["box_rho"].tobytes()
+    s.close()
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("size", [(37, 19, 23), (64, 3, 5), (1, 1, 1), (130, 2, 9)])
+def test_random_states_match_oracle(hip, oracle_cpu, dtype, size):
+    say("random", dtype.__name__, size)
+    rng = np.random.default_rng(hash(size) % 2**32)
+    kw = dict(inv_tau=1.53, lid_velocity=0.07, gravitation=(0.0009, -0.0013, 0.0004))
+    o = oracle_cpu.OracleSolver(size, (OBST,) * 6, dtype=dtype, **kw)
+    o.dd[:] = (rng.random(o.dd.size) * 0.08 + 0.01).astype(dtype)
+    o.flags[:] = rng.choice([OBST, FLUID, FLUID, FLUID, INJ, GHOST], size=o.n).astype(np.int32)
+    s = hip.LbmSolverHIP(size, (OBST,) * 6, PREC[dtype], kw["inv_tau"], kw["lid_velocity"],
+                         kw["gravitation"])
+    load_state(s, o)
+    nxh, nyh, nzh = o.size_h
+    boxes = [None, ((1, 0, 1), (max(1, nxh - 2), nyh, max(1, nzh - 1))),
+             ((0, nyh - 1, 0), (nxh, 1, nzh)), ((nxh - 1, 0, 0), (1, nyh, nzh))]
+    for n in range(8):
+        box = boxes[(n // 2) % len(boxes)]
+        args = () if box is None else box
+        if n & 1:
+            o.beta(*args)
+            s.simulationStepBeta(*args)
+        else:
+            o.alpha(*args)
+            s.simulationStepAlpha(*args)
+        same_state(s, o, f"step {n} box {box}")
+    s.close()
+
+
+def _cavity(hip, size, inv_tau, steps, split=(1, 1, 1)):
+    c = hip.CavityController(size, u_lid=0.05, reynolds=100.0, split=split)
+    assert np.float32(c.inv_tau) == np.float32(inv_tau)
+    for _ in range(steps):
+        c.computeNextStep()
+    c.sync()
+    return c
+
+
+def test_cavity16_matches_reference_golden(hip):
+    say("cavity16 200 steps")
+    g = np.load(os.path.join(GOLDEN, "cavity16.npz"))
+    c = _cavity(hip, (16, 16, 16), float(g["inv_tau"]), 200)
+    assert c.getVelocities().tobytes() == g["u"].tobytes()
+    assert c.getDensities().tobytes() == g["rho"].tobytes()
+    c.close()
+
+
+def test_config1_matches_reference_golden(hip):
+    """64^3 cells including the wall layer, lid 0.05, Re 100, 1000 steps."""
+    say("config1 62^3 interior 1000 steps")
+    with open(os.path.join(GOLDEN, "config1.json")) as f:
+        g = json.load(f)
+    c = _cavity(hip, (62, 62, 62), g["inv_tau"], 1000)
+    u = c.getVelocities()
+    rho = c.getDensities()
+    assert hashlib.sha256(u.tobytes()).hexdigest() == g["sha256_u"]
+    assert hashlib.sha256(rho.tobytes()).hexdigest() == g["sha256_rho"]
+    mid = np.load(os.path.join(GOLDEN, "config1_midplane.npz"))["u_mid_z31"]
+    assert (u[:, 31] == mid).all()
+    c.close()
+
+
+@pytest.mark.parametrize("split", [(2, 1, 1), (1, 2, 1), (1, 1, 2), (2, 2, 2)])
+def test_split_equals_monolithic(hip, split):
+    """main.cpp:121-303 validation semantic, with on-device face exchange."""
+    say("split", split)
+    size = (24, 24, 24)
+    mono = _cavity(hip, size, hip.cavity_parameters(size, 0.05, 100.0)["inv_tau"], 81)
+    parts = _cavity(hip, size, mono.inv_tau, 81, split)
+    assert np.abs(mono.getVelocities()).max() > 1e-3
+    assert mono.getVelocities().tobytes() == parts.getVelocities().tobytes()
+    assert mono.getDensities().tobytes() == parts.getDensities().tobytes()
+    mono.close()
+    parts.close()
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_box_copies_match_oracle(hip, oracle_cpu, dtype):
+    say("box copies", dtype.__name__)
+    rng = np.random.default_rng(11)
+    size = (11, 7, 5)
+    o = oracle_cpu.OracleSolver(size, (OBST,) * 6, dtype=dtype)
+    s = hip.LbmSolverHIP(size, (OBST,) * 6, PREC[dtype])
+    o.dd[:] = rng.random(o.dd.size).astype(dtype)
+    load_state(s, o)
+    for origin, box in [((0, 0, 0), (13, 9, 7)), ((2, 1, 3), (5, 6, 2)), ((12, 8, 6), (1, 1, 1))]:
+        assert s.getDensityDistributions(origin, box).tobytes() == \
+            o.getDensityDistributions(origin, box).tobytes()
+        src = rng.random(19 * int(np.prod(box))).astype(dtype)
+        for direction in (None, 0, 1, 2, 3, 4, 5):
+            s.setDensityDistributions(src, origin, box, direction)
+            o.setDensityDistributions(origin, box, src, direction)
+            assert s.getDensityDistributions((0, 0, 0), (13, 9, 7)).tobytes() == o.dd.tobytes()
+        uvals = rng.random(3 * int(np.prod(box))).astype(dtype)
+        s.setVelocities(uvals, origin, box)
+        assert s.getVelocities(origin, box).tobytes() == uvals.tobytes()
+        rvals = rng.random(int(np.prod(box))).astype(dtype)
+        s.setDensities(rvals, origin, box)
+        assert s.getDensities(origin, box).tobytes() == rvals.tobytes()
+        fl = rng.choice([1, 2, 4, 8], size=int(np.prod(box))).astype(np.int32)
+        s.setFlags(fl, origin, box)
+        assert (s.getFlags(origin, box) == fl).all()
+    # interior default boxes
+    assert s.getVelocities().size == 3 * 11 * 7 * 5
+    s.close()
+
+
+def test_box_errors(hip):
+    say("box errors")
+    s = hip.LbmSolverHIP((4, 4, 4))
+    for origin, box in [((-1, 0, 0), (1, 1, 1)), ((0, 0, 0), (7, 1, 1)), ((5, 5, 5), (1, 1, 1)),
+                        ((0, 0, 0), (0, 1, 1))]:
+        with pytest.raises(hip.LbmError) as e:
+            s.simulationStepAlpha(origin, box)
+        assert e.value.status == -2
+        with pytest.raises(hip.LbmError):
+            s.getDensityDistributions(origin, box)
+    with pytest.raises(hip.LbmError) as e:
+        s.setFlags(np.full(8, 3, np.int32), (1, 1, 1), (2, 2, 2))
+    assert e.value.status == -1
+    t = hip.LbmSolverHIP((4, 4, 4), store_densities=False, store_velocities=False)
+    with pytest.raises(hip.LbmError) as e:
+        t.getVelocities()
+    assert e.value.status == -7
+    s.close()
+    t.close()
+
+
+def test_pack_unpack_faces_match_oracle(hip, oracle_cpu):
+    import torch
+    say("pack/unpack faces")
+    rng = np.random.default_rng(12)
+    size = (9, 6, 7)
+    o = oracle_cpu.OracleSolver(size, (OBST,) * 6, dtype=np.float32)
+    s = hip.LbmSolverHIP(size, (OBST,) * 6, "f32")
+    o.dd[:] = rng.random(o.dd.size).astype(np.float32)
+    load_state(s, o)
+    for direction in range(6):
+        n = s.face_elements(direction)
+        for phase in (0, 1):
+            buf = torch.empty(n, dtype=torch.float32, device="cuda")
+            s.pack_face(direction, phase, buf)
+            s.sync()
+            origin, box = s.face_box(direction, 0 if phase == 0 else 1)
+            assert buf.cpu().numpy().tobytes() == o.getDensityDistributions(origin, box).tobytes()
+            src = rng.random(n).astype(np.float32)
+            s.unpack_face(direction, phase, torch.from_numpy(src).cuda())
+            s.sync()
+            origin, box = s.face_box(direction, 1 if phase == 0 else 0)
+            o.setDensityDistributions(origin, box, src, None if phase == 0 else direction)
+            assert s.getDensityDistributions((0, 0, 0), s.size_h).tobytes() == o.dd.tobytes()
+    s.close()
+
+
+@pytest.mark.parametrize("prec,tol", [("f32", 2e-6), ("f64", 1e-13)])
+def test_mass_is_conserved_in_closed_box(hip, prec, tol):
+    """No lid: the total of all populations (walls included) is invariant."""
+    say("mass", prec)
+    s = hip.LbmSolverHIP((20, 16, 12), precision=prec, inv_tau=1.7)
+    dtype = np.float64 if prec == "f64" else np.float32
+    rng = np.random.default_rng(13)
+    d = s.getDensityDistributions((1, 1, 1), (20, 16, 12))
+    d = (d * (1 + 0.02 * (rng.random(d.size) - 0.5))).astype(dtype)
+    s.setDensityDistributions(d, (1, 1, 1), (20, 16, 12))
+    m0 = s.reduce_box()[0]
+    for n in range(200):
+        (s.simulationStepBeta if n & 1 else s.simulationStepAlpha)()
+    m1 = s.reduce_box()[0]
+    assert abs(m1 - m0) / m0 < tol * 200
+    s.close()
+
+
+def test_residual_decreases_toward_steady_state(hip):
+    say("residual")
+    c = hip.CavityController((16, 16, 16), u_lid=0.05, reynolds=100.0)
+    s = next(iter(c.solvers.values()))
+    res = []
+    for n in range(2000):
+        c.computeNextStep()
+        if n % 200 == 199:
+            res.append(s.residual(layout=0))   # after a beta step
+    assert res[0] == float("inf")
+    assert all(np.isfinite(res[1:]))
+    assert res[-1] < res[1] * 0.5
+    c.close()
