@@ -194,7 +194,7 @@ def test_box_copies_match_oracle(hip, oracle_cpu, dtype):
 def test_box_errors(hip):
     say("box errors")
     s = hip.LbmSolverHIP((4, 4, 4))
-    for origin, box in [((-1, 0, 0), (1, 1, 1)), ((0, 0, 0), (7, 1, 1)), ((5, 5, 5), (1, 1, 1)),
+    for origin, box in [((-1, 0, 0), (1, 1, 1)), ((0, 0, 0), (7, 1, 1)), ((6, 5, 5), (1, 1, 1)),
                         ((0, 0, 0), (0, 1, 1))]:
         with pytest.raises(hip.LbmError) as e:
             s.simulationStepAlpha(origin, box)
@@ -213,7 +213,7 @@ def test_box_errors(hip):
 
 
 def test_pack_unpack_faces_match_oracle(hip, oracle_cpu):
-    import torch
+    from hiprt import DeviceBuffer
     say("pack/unpack faces")
     rng = np.random.default_rng(12)
     size = (9, 6, 7)
@@ -224,14 +224,17 @@ def test_pack_unpack_faces_match_oracle(hip, oracle_cpu):
     for direction in range(6):
         n = s.face_elements(direction)
         for phase in (0, 1):
-            buf = torch.empty(n, dtype=torch.float32, device="cuda")
+            buf = DeviceBuffer(4 * n)
             s.pack_face(direction, phase, buf)
             s.sync()
             origin, box = s.face_box(direction, 0 if phase == 0 else 1)
-            assert buf.cpu().numpy().tobytes() == o.getDensityDistributions(origin, box).tobytes()
+            packed = buf.download(np.float32, n)
+            assert packed.tobytes() == o.getDensityDistributions(origin, box).tobytes()
             src = rng.random(n).astype(np.float32)
-            s.unpack_face(direction, phase, torch.from_numpy(src).cuda())
+            buf.upload(src)
+            s.unpack_face(direction, phase, buf)
             s.sync()
+            buf.free()
             origin, box = s.face_box(direction, 1 if phase == 0 else 0)
             o.setDensityDistributions(origin, box, src, None if phase == 0 else direction)
             assert s.getDensityDistributions((0, 0, 0), s.size_h).tobytes() == o.dd.tobytes()
