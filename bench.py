@@ -155,6 +155,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-copy", action="store_true")
+    ap.add_argument("--launch-alpha", type=int, default=None,
+                    help="LBM_LAUNCH_* bits for the alpha kernel (default: library default)")
+    ap.add_argument("--launch-beta", type=int, default=None)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -180,6 +183,14 @@ def main():
                             precision=args.precision, store_outputs=False, device=device,
                             params=params, stream=stream.value)
     solver = next(iter(c.solvers.values()))
+    for k, opt in ((0, args.launch_alpha), (1, args.launch_beta)):
+        if opt is not None:
+            check(L.lbm_set_launch_options(solver.handle, k, opt))
+    launch_opts = []
+    for k in (0, 1):
+        v = ctypes.c_int()
+        check(L.lbm_get_launch_options(solver.handle, k, ctypes.byref(v)))
+        launch_opts.append(v.value)
     halo = not args.no_halo
     shadow = None
     bufs = []
@@ -289,6 +300,7 @@ def main():
             "precision": args.precision,
             "halo_exchange": halo,
             "parallelism": "single GPU (one rank's subdomain)",
+            "launch_options": {"alpha": launch_opts[0], "beta": launch_opts[1]},
         },
         "roofline": {
             "bound": "hbm",

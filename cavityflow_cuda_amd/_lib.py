@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "libcavityflow_hip.so")
 OBSTACLE, FLUID, VELOCITY_INJECTION, GHOST_LAYER = 1, 2, 4, 8
 LEFT, RIGHT, BOTTOM, TOP, BACK, FRONT = range(6)
 F32, F64, F16S, BF16S = 0, 1, 2, 3
+LAUNCH_NONTEMPORAL, LAUNCH_XCD_REMAP = 1, 2
 LBM_OK = 0
 
 EXPORTED = (
@@ -25,6 +26,7 @@ EXPORTED = (
     "lbm_get_velocities", "lbm_set_velocities", "lbm_get_densities", "lbm_set_densities",
     "lbm_face_box", "lbm_face_elements", "lbm_pack_face", "lbm_unpack_face",
     "lbm_exchange_face", "lbm_device_alloc", "lbm_device_free",
+    "lbm_set_launch_options", "lbm_get_launch_options",
     "lbm_reduce_box", "lbm_residual", "lbm_sync", "lbm_last_error", "lbm_status_string",
     "lbm_abi_version",
 )
@@ -132,6 +134,8 @@ def lib() -> ctypes.CDLL:
         "lbm_exchange_face": ([P, I, I, P, P], I),
         "lbm_device_alloc": ([I, ctypes.c_size_t, ctypes.POINTER(P)], I),
         "lbm_device_free": ([I, P], I),
+        "lbm_set_launch_options": ([P, I, I], I),
+        "lbm_get_launch_options": ([P, I, ctypes.POINTER(I)], I),
         "lbm_reduce_box": ([P, i3, i3, ctypes.POINTER(ctypes.c_double), P], I),
         "lbm_residual": ([P, I, ctypes.POINTER(ctypes.c_double), P], I),
         "lbm_sync": ([P, P], I),

@@ -314,6 +314,8 @@ int lbm_create(const lbm_params* params, lbm_solver** out)
     s->stride = (n + 255) / 256 * 256;
     s->elem_bytes = eb;
     s->out_bytes = p.precision == LBM_F64 ? 8 : 4;
+    s->launch[0] = LBM_LAUNCH_NONTEMPORAL;
+    s->launch[1] = LBM_LAUNCH_XCD_REMAP;
     // round inv_tau / lid / gravity to the arithmetic type once
     if (p.precision != LBM_F64) {
         s->p.inv_tau = (float)p.inv_tau;
@@ -388,6 +390,27 @@ int lbm_get_device_arrays(const lbm_solver* s, lbm_device_arrays* out)
         out->size_h[i] = s->size_h[i];
     out->precision = s->p.precision;
     out->element_bytes = s->elem_bytes;
+    return LBM_OK;
+}
+
+int lbm_set_launch_options(lbm_solver* s, int kernel, int options)
+{
+    if (!s || kernel < 0 || kernel > 1 ||
+        (options & ~(LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_XCD_REMAP))) {
+        set_error("lbm_set_launch_options: invalid argument");
+        return LBM_ERR_ARG;
+    }
+    s->launch[kernel] = options;
+    return LBM_OK;
+}
+
+int lbm_get_launch_options(const lbm_solver* s, int kernel, int* options)
+{
+    if (!s || !options || kernel < 0 || kernel > 1) {
+        set_error("lbm_get_launch_options: invalid argument");
+        return LBM_ERR_ARG;
+    }
+    *options = s->launch[kernel];
     return LBM_OK;
 }
 

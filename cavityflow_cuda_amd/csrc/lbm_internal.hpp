@@ -53,6 +53,10 @@ struct StepArgs {
     FastDiv div_bx, div_by;
     C inv_tau, lid, g0, g1, g2;
     int store_rho, store_u;
+    int xcd;          // renumber workgroups per XCD
+    // byte address of plane l, shifted by the neighbour offset of direction l
+    // for beta (0 for alpha): uniform per-plane bases for 32-bit lane offsets
+    const char* plane[19];
 };
 
 }  // namespace lbm
@@ -73,6 +77,7 @@ struct lbm_solver {
     double* partials;  // reduction partials (kReduceBlocks * 5) followed by 5 results
     void* residual_prev;  // momentum saved by lbm_residual, 3 planes of the compute type
     int residual_valid;
+    int launch[2];     // LBM_LAUNCH_* options of the alpha and beta kernels
 };
 
 namespace lbm {

@@ -43,6 +43,53 @@ __device__ __forceinline__ uint32_t box_cell(const StepArgs<C>& a, uint32_t gid)
     return (uint32_t)((a.oz + (int32_t)z) * a.nxy + (a.oy + (int32_t)y) * a.nx + a.ox + (int32_t)x);
 }
 
+// Population accessors. Acc32 keeps one uniform base pointer per plane (the
+// beta neighbour shift is folded into it) and one 32-bit byte offset per lane
+// shared by all 19 planes, so every access is a saddr global load/store with
+// no per-plane vector address arithmetic; it needs cells * sizeof(S) < 2^32.
+// Acc64 uses a 64-bit element index per plane and, with WRAP, the reference's
+// linear-index wrap (domainWrap, CLbmBetaCPU.cpp:44-50).
+// The plane bases come precomputed in the kernel arguments: opaque uniform
+// values the compiler cannot fold into per-lane pointer chains.
+template <class S, bool NT>
+struct Acc32 {
+    const char* const* plane;
+    uint32_t off;
+    __device__ __forceinline__ S load(int l, int32_t) const
+    {
+        const S* p = (const S*)(plane[l] + off);
+        if (NT)
+            return __builtin_nontemporal_load(p);
+        return *p;
+    }
+    __device__ __forceinline__ void store(int l, int32_t, S v) const
+    {
+        S* p = (S*)(const_cast<char*>(plane[l]) + off);
+        if (NT)
+            __builtin_nontemporal_store(v, p);
+        else
+            *p = v;
+    }
+};
+
+template <class S, bool WRAP>
+struct Acc64 {
+    S* dd;
+    int64_t stride;
+    int32_t id, n;
+    __device__ __forceinline__ int64_t index(int l, int32_t delta) const
+    {
+        int32_t j = id + delta;
+        if (WRAP) {
+            j = (j >= n) ? j - n : j;
+            j = (j < 0) ? j + n : j;
+        }
+        return (int64_t)l * stride + j;
+    }
+    __device__ __forceinline__ S load(int l, int32_t delta) const { return dd[index(l, delta)]; }
+    __device__ __forceinline__ void store(int l, int32_t delta, S v) const { dd[index(l, delta)] = v; }
+};
+
 // ------------------------------------------------------------------ init ---
 
 // lbm_init.cu:41-208 / CLbmInitCPU.cpp:37-171: flags from the six face
@@ -126,20 +173,20 @@ __global__ __launch_bounds__(kBlock) void k_init(StepArgs<typename P::C> a, int 
 // One cell of CLbmAlphaCPU::alphaKernelCPU (CLbmAlphaCPU.cpp:75-494). The
 // reference's `tmp` is a #define for `rho` (:199): every gravity term
 // overwrites rho and the stored density is the last gravity term.
-template <class P>
-__device__ __forceinline__ void alpha_cell(const StepArgs<typename P::C>& a, uint32_t id)
+template <class P, class A>
+__device__ __forceinline__ void alpha_cell(const StepArgs<typename P::C>& a, uint32_t id,
+                                           const A& acc)
 {
     using C = typename P::C;
-    using S = typename P::S;
-    S* __restrict__ dd = (S*)a.dd;
     const int flag = a.flags[id];
-    if (flag == kGhostLayer)
+    // GHOST_LAYER is skipped (:75-76); OBSTACLE without outputs does nothing
+    if (flag == kGhostLayer || (flag == kObstacle && !a.store_u && !a.store_rho))
         return;
 
     C d[kQ];
 #pragma unroll
     for (int l = 0; l < kQ; l++)
-        d[l] = P::load(dd[(int64_t)l * a.stride + id], l);
+        d[l] = P::load(acc.load(l, 0), l);
     C rho, ux, uy, uz;
     moments(d, rho, ux, uy, uz);
 
@@ -234,7 +281,7 @@ __device__ __forceinline__ void alpha_cell(const StepArgs<typename P::C>& a, uin
         // direction l goes to slot opp(l) of this cell
 #pragma unroll
         for (int l = 0; l < kQ; l++)
-            dd[(int64_t)opp(l) * a.stride + id] = P::store(e[l], opp(l));
+            acc.store(opp(l), 0, P::store(e[l], opp(l)));
     } else if (flag == kObstacle) {
         // no stores: the A-A layout makes bounce-back implicit (:330-343)
         if (a.store_u) {
@@ -253,13 +300,22 @@ __device__ __forceinline__ void alpha_cell(const StepArgs<typename P::C>& a, uin
         a.rho[id] = rho;
 }
 
-template <class P, bool FULL>
+template <class P, bool FULL, bool OFF32, bool NT>
 __global__ __launch_bounds__(kBlock) void k_alpha(StepArgs<typename P::C> a)
 {
-    const uint32_t gid = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    using S = typename P::S;
+    const uint32_t blk = a.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const uint32_t gid = blk * kBlock + threadIdx.x;
     if (gid >= a.cells)
         return;
-    alpha_cell<P>(a, FULL ? gid : box_cell(a, gid));
+    const uint32_t id = FULL ? gid : box_cell(a, gid);
+    if (OFF32) {
+        const Acc32<S, NT> acc{a.plane, id * (uint32_t)sizeof(S)};
+        alpha_cell<P>(a, id, acc);
+    } else {
+        const Acc64<S, false> acc{(S*)a.dd, a.stride, (int32_t)id, a.n};
+        alpha_cell<P>(a, id, acc);
+    }
 }
 
 // ------------------------------------------------------------------ beta ---
@@ -269,29 +325,21 @@ __global__ __launch_bounds__(kBlock) void k_alpha(StepArgs<typename P::C> a)
 // rho - 1.5 |j|^2. OBSTACLE swaps opposite pairs, which writes back exactly
 // what was read, so its stores are skipped. GHOST_LAYER passes values through.
 // Neighbour indices wrap modulo the cell count (domainWrap, :44-50).
-template <class P, bool WRAP>
-__device__ __forceinline__ void beta_cell(const StepArgs<typename P::C>& a, uint32_t id)
+template <class P, class A>
+__device__ __forceinline__ void beta_cell(const StepArgs<typename P::C>& a, uint32_t id,
+                                          const A& acc)
 {
     using C = typename P::C;
-    using S = typename P::S;
-    S* __restrict__ dd = (S*)a.dd;
     const int flag = a.flags[id];
+    // OBSTACLE writes back what it reads; without outputs it does nothing
+    if (flag == kObstacle && !a.store_u && !a.store_rho)
+        return;
 
-    int32_t pos[kQ];
-#pragma unroll
-    for (int l = 0; l < kQ; l++) {
-        int32_t j = (int32_t)id + (cx(l) + cy(l) * a.nx + cz(l) * a.nxy);
-        if (WRAP) {
-            j = (j >= a.n) ? j - a.n : j;
-            j = (j < 0) ? j + a.n : j;
-        }
-        pos[l] = j;
-    }
     // plane l at id + c_l holds direction opp(l) arriving at this cell
     C d[kQ];
 #pragma unroll
     for (int l = 0; l < kQ; l++)
-        d[opp(l)] = P::load(dd[(int64_t)l * a.stride + pos[l]], l);
+        d[opp(l)] = P::load(acc.load(l, cx(l) + cy(l) * a.nx + cz(l) * a.nxy), l);
 
     C rho, ux, uy, uz;
     moments(d, rho, ux, uy, uz);
@@ -387,7 +435,7 @@ __device__ __forceinline__ void beta_cell(const StepArgs<typename P::C>& a, uint
         // direction l goes to slot l of neighbour id + c_l
 #pragma unroll
         for (int l = 0; l < kQ; l++)
-            dd[(int64_t)l * a.stride + pos[l]] = P::store(d[l], l);
+            acc.store(l, cx(l) + cy(l) * a.nx + cz(l) * a.nxy, P::store(d[l], l));
     }
 
     if (a.store_u) {
@@ -399,10 +447,11 @@ __device__ __forceinline__ void beta_cell(const StepArgs<typename P::C>& a, uint
         a.rho[id] = rho;
 }
 
-template <class P, bool FULL>
+template <class P, bool FULL, bool OFF32, bool NT>
 __global__ __launch_bounds__(kBlock) void k_beta(StepArgs<typename P::C> a)
 {
-    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    using S = typename P::S;
+    const uint32_t blk = a.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     const uint32_t gid = blk * kBlock + threadIdx.x;
     // One decision per workgroup: neighbours of every cell of this block stay
     // inside [0, n) unless the block touches the first or last slab.
@@ -415,10 +464,16 @@ __global__ __launch_bounds__(kBlock) void k_beta(StepArgs<typename P::C> a)
     if (gid >= a.cells)
         return;
     const uint32_t id = FULL ? gid : box_cell(a, gid);
-    if (wrap)
-        beta_cell<P, true>(a, id);
-    else
-        beta_cell<P, false>(a, id);
+    if (wrap) {
+        const Acc64<S, true> acc{(S*)a.dd, a.stride, (int32_t)id, a.n};
+        beta_cell<P>(a, id, acc);
+    } else if (OFF32) {
+        const Acc32<S, NT> acc{a.plane, id * (uint32_t)sizeof(S)};
+        beta_cell<P>(a, id, acc);
+    } else {
+        const Acc64<S, false> acc{(S*)a.dd, a.stride, (int32_t)id, a.n};
+        beta_cell<P>(a, id, acc);
+    }
 }
 
 // -------------------------------------------------------------- launches ---
@@ -449,6 +504,9 @@ static StepArgs<typename P::C> make_args(const lbm_solver* s, const Box& b)
     a.g2 = (C)s->p.gravitation[2];
     a.store_rho = s->p.store_densities;
     a.store_u = s->p.store_velocities;
+    a.xcd = 0;
+    for (int l = 0; l < kQ; l++)
+        a.plane[l] = (const char*)s->dd + (int64_t)l * s->stride * (int64_t)sizeof(typename P::S);
     return a;
 }
 
@@ -471,15 +529,46 @@ static hipError_t init_t(const lbm_solver* s, hipStream_t st)
     return hipGetLastError();
 }
 
+// 32-bit byte offsets reach every element of a plane
+static bool off32(const lbm_solver* s)
+{
+    return (uint64_t)s->stride * (uint64_t)s->elem_bytes < (1ull << 32);
+}
+
+template <class P, template <class, bool, bool, bool> class K, bool FULL>
+static void launch_variant(const lbm_solver* s, const StepArgs<typename P::C>& a, int opts,
+                           hipStream_t st)
+{
+    const uint32_t blocks = (a.cells + kBlock - 1) / kBlock;
+    if (!off32(s))
+        hipLaunchKernelGGL((K<P, FULL, false, false>::kernel), dim3(blocks), dim3(kBlock), 0,
+                           st, a);
+    else if (opts & LBM_LAUNCH_NONTEMPORAL)
+        hipLaunchKernelGGL((K<P, FULL, true, true>::kernel), dim3(blocks), dim3(kBlock), 0, st,
+                           a);
+    else
+        hipLaunchKernelGGL((K<P, FULL, true, false>::kernel), dim3(blocks), dim3(kBlock), 0,
+                           st, a);
+}
+
+template <class P, bool FULL, bool OFF32, bool NT>
+struct AlphaK {
+    static constexpr auto kernel = k_alpha<P, FULL, OFF32, NT>;
+};
+template <class P, bool FULL, bool OFF32, bool NT>
+struct BetaK {
+    static constexpr auto kernel = k_beta<P, FULL, OFF32, NT>;
+};
+
 template <class P>
 static hipError_t alpha_t(const lbm_solver* s, const Box& b, hipStream_t st)
 {
     auto a = make_args<P>(s, b);
-    const uint32_t blocks = (a.cells + kBlock - 1) / kBlock;
+    a.xcd = (s->launch[0] & LBM_LAUNCH_XCD_REMAP) ? 1 : 0;
     if (is_full(s, b))
-        hipLaunchKernelGGL((k_alpha<P, true>), dim3(blocks), dim3(kBlock), 0, st, a);
+        launch_variant<P, AlphaK, true>(s, a, s->launch[0], st);
     else
-        hipLaunchKernelGGL((k_alpha<P, false>), dim3(blocks), dim3(kBlock), 0, st, a);
+        launch_variant<P, AlphaK, false>(s, a, s->launch[0], st);
     return hipGetLastError();
 }
 
@@ -487,11 +576,15 @@ template <class P>
 static hipError_t beta_t(const lbm_solver* s, const Box& b, hipStream_t st)
 {
     auto a = make_args<P>(s, b);
-    const uint32_t blocks = (a.cells + kBlock - 1) / kBlock;
+    a.xcd = (s->launch[1] & LBM_LAUNCH_XCD_REMAP) ? 1 : 0;
+    // beta reads and writes plane l at id + c_l: fold the shift into the base
+    for (int l = 0; l < kQ; l++)
+        a.plane[l] += (int64_t)(cx(l) + cy(l) * a.nx + cz(l) * a.nxy) *
+                      (int64_t)sizeof(typename P::S);
     if (is_full(s, b))
-        hipLaunchKernelGGL((k_beta<P, true>), dim3(blocks), dim3(kBlock), 0, st, a);
+        launch_variant<P, BetaK, true>(s, a, s->launch[1], st);
     else
-        hipLaunchKernelGGL((k_beta<P, false>), dim3(blocks), dim3(kBlock), 0, st, a);
+        launch_variant<P, BetaK, false>(s, a, s->launch[1], st);
     return hipGetLastError();
 }
 

@@ -161,6 +161,18 @@ int lbm_simulation_step_alpha(lbm_solver* solver, const int origin[3], const int
 int lbm_simulation_step_beta(lbm_solver* solver, const int origin[3], const int size[3],
                              hipStream_t stream);
 
+/* Launch variants of the step kernels; they change speed, never results.
+ * kernel: 0 = alpha, 1 = beta. Defaults: alpha LBM_LAUNCH_NONTEMPORAL,
+ * beta LBM_LAUNCH_XCD_REMAP (chosen by measurement on MI355X). No reference
+ * counterpart: the reference takes its block shapes from the XML
+ * configuration (default.xml:93-115). */
+enum lbm_launch_option {
+    LBM_LAUNCH_NONTEMPORAL = 1,  /* nontemporal population loads/stores */
+    LBM_LAUNCH_XCD_REMAP = 2     /* one contiguous range of cells per XCD */
+};
+int lbm_set_launch_options(lbm_solver* solver, int kernel, int options);
+int lbm_get_launch_options(const lbm_solver* solver, int kernel, int* options);
+
 /* ---- box copies (halo exchange, checkpoint, extraction) ------------------ */
 
 /* getDensityDistributions(origin, size, dst[, stream]) / (dst)
