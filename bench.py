@@ -151,6 +151,8 @@ def main():
     ap.add_argument("--u-lid", type=float, default=0.1)
     ap.add_argument("--precision", default="f32", choices=sorted(BYTES_PER_POP))
     ap.add_argument("--no-halo", action="store_true")
+    ap.add_argument("--no-fused-pack", action="store_true",
+                    help="pack the x faces with separate kernels instead of inside the steps")
     ap.add_argument("--cpu-sample-steps", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--skip-cpu", action="store_true")
@@ -204,11 +206,17 @@ def main():
             check(L.lbm_device_alloc(device, solver.face_elements(d) * elem, ctypes.byref(p)))
             bufs.append(p)
 
+    fused = halo and not args.no_fused_pack
+    if fused:
+        # the step kernels pack the LEFT/RIGHT faces themselves
+        check(L.lbm_set_step_pack(solver.handle, bufs[0], bufs[1]))
+
     def halo_exchange(phase):
         # reference order: x faces, then y, then z (CController.cpp:369-570)
         for axis in range(3):
-            for d in (2 * axis, 2 * axis + 1):
-                check(L.lbm_pack_face(solver.handle, d, phase, bufs[d], stream))
+            if not (fused and axis == 0):
+                for d in (2 * axis, 2 * axis + 1):
+                    check(L.lbm_pack_face(solver.handle, d, phase, bufs[d], stream))
             for d in (2 * axis, 2 * axis + 1):
                 check(L.lbm_unpack_face(shadow.handle, d ^ 1, phase, bufs[d], stream))
 
@@ -299,6 +307,7 @@ def main():
             "tau": round(params["tau"], 6),
             "precision": args.precision,
             "halo_exchange": halo,
+            "fused_x_pack": fused,
             "parallelism": "single GPU (one rank's subdomain)",
             "launch_options": {"alpha": launch_opts[0], "beta": launch_opts[1]},
         },

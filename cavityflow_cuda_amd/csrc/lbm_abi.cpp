@@ -668,6 +668,17 @@ int lbm_unpack_face(lbm_solver* s, int direction, int phase, const void* dev_buf
     return LBM_OK;
 }
 
+int lbm_set_step_pack(lbm_solver* s, void* left_buf, void* right_buf)
+{
+    if (!s) {
+        set_error("lbm_set_step_pack: solver is NULL");
+        return LBM_ERR_ARG;
+    }
+    s->xpack[0] = left_buf;
+    s->xpack[1] = right_buf;
+    return LBM_OK;
+}
+
 int lbm_exchange_face(lbm_solver* from, int direction, int phase, lbm_solver* to,
                       hipStream_t stream)
 {

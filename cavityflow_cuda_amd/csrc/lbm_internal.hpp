@@ -57,6 +57,12 @@ struct StepArgs {
     // byte address of plane l, shifted by the neighbour offset of direction l
     // for beta (0 for alpha): uniform per-plane bases for 32-bit lane offsets
     const char* plane[19];
+    // fused x-face pack: dense face buffers [slot][z * ny + y] (LEFT, RIGHT)
+    C* xpack[2];
+    int xpack_on;
+    FastDiv div_nx;
+    int32_t ny;
+    int32_t fcells;   // ny * nz
 };
 
 }  // namespace lbm
@@ -78,6 +84,7 @@ struct lbm_solver {
     void* residual_prev;  // momentum saved by lbm_residual, 3 planes of the compute type
     int residual_valid;
     int launch[2];     // LBM_LAUNCH_* options of the alpha and beta kernels
+    void* xpack[2];    // fused x-face pack buffers (LEFT, RIGHT), NULL when off
 };
 
 namespace lbm {

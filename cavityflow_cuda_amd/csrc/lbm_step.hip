@@ -168,6 +168,65 @@ __global__ __launch_bounds__(kBlock) void k_init(StepArgs<typename P::C> a, int 
         a.rho[id] = rho;
 }
 
+// ------------------------------------------------------- fused x packing ---
+
+// Whether the cell lies in the LEFT (x = 1) / RIGHT (x = nx - 2) send plane;
+// row = z * ny + y is its index in the dense face buffers.
+template <class C>
+__device__ __forceinline__ bool xpack_where(const StepArgs<C>& a, uint32_t id, uint32_t& row,
+                                            bool& pl, bool& pr)
+{
+    pl = pr = false;
+    row = 0;
+    if (!a.xpack_on)
+        return false;
+    row = a.div_nx.div(id);
+    const int x = (int)(id - row * (uint32_t)a.nx);
+    pl = x == 1 && a.xpack[0] != nullptr;
+    pr = x == a.nx - 2 && a.xpack[1] != nullptr;
+    return pl || pr;
+}
+
+// After alpha: the 19 slot values of a send-plane cell, in the layout
+// lbm_pack_face writes ([slot][row]); values pass through the storage type so
+// 16-bit storage packs exactly what was stored.
+template <class P>
+__device__ __forceinline__ void xpack_alpha(const StepArgs<typename P::C>& a, uint32_t row,
+                                            bool pl, bool pr, const typename P::C (&v)[kQ])
+{
+    using C = typename P::C;
+#pragma unroll
+    for (int s = 0; s < kQ; s++) {
+        const C w = P::load(P::store(v[s], s), s);
+        if (pl)
+            a.xpack[0][(int64_t)s * a.fcells + row] = w;
+        if (pr)
+            a.xpack[1][(int64_t)s * a.fcells + row] = w;
+    }
+}
+
+// After beta: the populations that cross the x faces. From a LEFT send-plane
+// cell the directions with c_x = -1 land in the LEFT halo plane, from a RIGHT
+// one those with c_x = +1 land in the RIGHT halo plane; each is written at
+// its landing cell's row (reference linear wrap), i.e. exactly the entries
+// of the halo-plane pack that the neighbour's direction-filtered unpack reads.
+template <class P>
+__device__ __forceinline__ void xpack_beta(const StepArgs<typename P::C>& a, uint32_t row,
+                                           bool pl, bool pr, const typename P::C (&v)[kQ])
+{
+    using C = typename P::C;
+#pragma unroll
+    for (int l = 0; l < kQ - 1; l++) {
+        const bool side = cx(l) < 0 ? pl : (cx(l) > 0 ? pr : false);
+        if (cx(l) != 0 && side) {
+            int32_t t = (int32_t)row + cy(l) + cz(l) * a.ny;
+            t = t < 0 ? t + a.fcells : t;
+            t = t >= a.fcells ? t - a.fcells : t;
+            a.xpack[cx(l) < 0 ? 0 : 1][(int64_t)l * a.fcells + t] = P::load(P::store(v[l], l), l);
+        }
+    }
+}
+
 // ----------------------------------------------------------------- alpha ---
 
 // One cell of CLbmAlphaCPU::alphaKernelCPU (CLbmAlphaCPU.cpp:75-494). The
@@ -179,14 +238,21 @@ __device__ __forceinline__ void alpha_cell(const StepArgs<typename P::C>& a, uin
 {
     using C = typename P::C;
     const int flag = a.flags[id];
+    uint32_t row;
+    bool pl, pr;
+    const bool pack = xpack_where(a, id, row, pl, pr);
     // GHOST_LAYER is skipped (:75-76); OBSTACLE without outputs does nothing
-    if (flag == kGhostLayer || (flag == kObstacle && !a.store_u && !a.store_rho))
+    if (!pack && (flag == kGhostLayer || (flag == kObstacle && !a.store_u && !a.store_rho)))
         return;
 
     C d[kQ];
 #pragma unroll
     for (int l = 0; l < kQ; l++)
         d[l] = P::load(acc.load(l, 0), l);
+    if (flag == kGhostLayer) {   // reached only to pack its unchanged slots
+        xpack_alpha<P>(a, row, pl, pr, d);
+        return;
+    }
     C rho, ux, uy, uz;
     moments(d, rho, ux, uy, uz);
 
@@ -282,8 +348,17 @@ __device__ __forceinline__ void alpha_cell(const StepArgs<typename P::C>& a, uin
 #pragma unroll
         for (int l = 0; l < kQ; l++)
             acc.store(opp(l), 0, P::store(e[l], opp(l)));
+        if (pack) {
+            C out[kQ];
+#pragma unroll
+            for (int s2 = 0; s2 < kQ; s2++)
+                out[s2] = e[opp(s2)];
+            xpack_alpha<P>(a, row, pl, pr, out);
+        }
     } else if (flag == kObstacle) {
         // no stores: the A-A layout makes bounce-back implicit (:330-343)
+        if (pack)
+            xpack_alpha<P>(a, row, pl, pr, d);
         if (a.store_u) {
             ux = (C)0;
             uy = (C)0;
@@ -331,8 +406,11 @@ __device__ __forceinline__ void beta_cell(const StepArgs<typename P::C>& a, uint
 {
     using C = typename P::C;
     const int flag = a.flags[id];
+    uint32_t row;
+    bool pl, pr;
+    const bool pack = xpack_where(a, id, row, pl, pr);
     // OBSTACLE writes back what it reads; without outputs it does nothing
-    if (flag == kObstacle && !a.store_u && !a.store_rho)
+    if (!pack && flag == kObstacle && !a.store_u && !a.store_rho)
         return;
 
     // plane l at id + c_l holds direction opp(l) arriving at this cell
@@ -424,6 +502,14 @@ __device__ __forceinline__ void beta_cell(const StepArgs<typename P::C>& a, uint
         d[17] = eq_a1(uz, vela2, rho);
         d[18] = eq_18(rho);
     } else if (flag == kObstacle) {
+        if (pack) {
+            // the swap writes back the value read from plane l (:302-311)
+            C out[kQ];
+#pragma unroll
+            for (int l = 0; l < kQ; l++)
+                out[l] = d[opp(l)];
+            xpack_beta<P>(a, row, pl, pr, out);
+        }
         if (a.store_u) {
             ux = (C)0;
             uy = (C)0;
@@ -436,6 +522,8 @@ __device__ __forceinline__ void beta_cell(const StepArgs<typename P::C>& a, uint
 #pragma unroll
         for (int l = 0; l < kQ; l++)
             acc.store(l, cx(l) + cy(l) * a.nx + cz(l) * a.nxy, P::store(d[l], l));
+        if (pack)
+            xpack_beta<P>(a, row, pl, pr, d);
     }
 
     if (a.store_u) {
@@ -505,6 +593,12 @@ static StepArgs<typename P::C> make_args(const lbm_solver* s, const Box& b)
     a.store_rho = s->p.store_densities;
     a.store_u = s->p.store_velocities;
     a.xcd = 0;
+    a.xpack[0] = (C*)s->xpack[0];
+    a.xpack[1] = (C*)s->xpack[1];
+    a.xpack_on = (s->xpack[0] || s->xpack[1]) ? 1 : 0;
+    a.div_nx = FastDiv((uint32_t)s->size_h[0]);
+    a.ny = s->size_h[1];
+    a.fcells = s->size_h[1] * s->size_h[2];
     for (int l = 0; l < kQ; l++)
         a.plane[l] = (const char*)s->dd + (int64_t)l * s->stride * (int64_t)sizeof(typename P::S);
     return a;

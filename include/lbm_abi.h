@@ -239,6 +239,20 @@ int lbm_pack_face(lbm_solver* solver, int direction, int phase, void* dev_buf,
 int lbm_unpack_face(lbm_solver* solver, int direction, int phase, const void* dev_buf,
                     hipStream_t stream);
 
+/* Fused halo packing of the two x faces into caller-owned device buffers of
+ * lbm_face_elements(LEFT) elements each (NULL disables a side, both NULL
+ * turns it off). While set, every whole-domain or box step also writes:
+ *   alpha: all 19 populations of the send plane (x = 1 / x = nx_h - 2) into
+ *          the buffer, exactly what lbm_pack_face(phase 0) produces;
+ *   beta : the 5 populations that cross the face into the halo plane, at
+ *          their halo-plane positions, i.e. the entries of
+ *          lbm_pack_face(phase 1) that the neighbour's
+ *          lbm_unpack_face(phase 1) reads (the other 14 are not written).
+ * The x faces come first in the reference's exchange order
+ * (CController.cpp:369-436, 807-874), so packing them inside the step is
+ * exact; y and z faces keep lbm_pack_face. No reference counterpart. */
+int lbm_set_step_pack(lbm_solver* solver, void* left_buf, void* right_buf);
+
 /* Single-GPU stand-in for the neighbour send/recv of one face: packs `from`'s
  * face `direction` (phase as in lbm_pack_face) through `from`'s device
  * staging buffer and unpacks it into `to` across the opposite face, e.g.
