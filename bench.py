@@ -141,6 +141,77 @@ def cpu_baseline(size, params, sample_steps, threads_all):
     return out
 
 
+def precision_workload(args, cf, L, hip, stream):
+    """Shifted 16-bit storage against FP64 on the same cavity: 512^3 interior,
+    lid 0.1, Re drawn from the seed in [400, 3200]; each run stops at a
+    residual ||u^n - u^(n-100)|| / ||u^n|| <= 1e-6 or after --steps steps."""
+    size = (args.size,) * 3
+    rng = np.random.default_rng(args.seed)
+    re = float(rng.uniform(400.0, 3200.0))
+    cells = int(np.prod(size))
+    runs = {}
+    j64 = None
+    for prec in ("f64", "f32", "f16s", "bf16s"):
+        params = cf.cavity_parameters(size, args.u_lid, re, prec)
+        c = cf.CavityController(size, u_lid=args.u_lid, reynolds=re, precision=prec,
+                                store_outputs=False, params=params, stream=stream.value)
+        s = next(iter(c.solvers.values()))
+        print(f"[precision] {prec} Re={re:.1f} tau={params['tau']:.5f}", file=sys.stderr,
+              flush=True)
+        e0, e1 = hip.event(), hip.event()
+        hip.sync(stream)
+        hip.record(e0, stream)
+        w0 = time.perf_counter()
+        last = w0
+        steps, residual = 0, float("inf")
+        while steps < args.steps:
+            for _ in range(100):
+                c.computeNextStep()
+            steps += 100
+            residual = s.residual(0, stream.value)   # even count: last step was beta
+            if time.perf_counter() - last > 20:
+                last = time.perf_counter()
+                print(f"[precision] {prec} step {steps} residual {residual:.3e}",
+                      file=sys.stderr, flush=True)
+            if residual <= 1e-6:
+                break
+        hip.record(e1, stream)
+        ms = hip.elapsed_ms(e0, e1)
+        m = s.getMoments(0, stream=stream.value).reshape(4, -1)
+        j = m[1:].astype(np.float64)
+        rec = dict(steps=steps, residual=residual, mlups=round(cells * steps / (ms * 1e-3) / 1e6, 1),
+                   tau=params["tau"])
+        if prec == "f64":
+            j64 = j
+            norm = float(np.sqrt((j64 ** 2).sum()))
+            rec["u_rms"] = float(np.sqrt((j64 ** 2).sum() / cells))
+        else:
+            d = j - j64
+            rec["l2_rel"] = float(np.sqrt((d ** 2).sum()) / norm)
+            rec["rms_abs"] = float(np.sqrt((d ** 2).sum() / cells))
+            rec["linf_abs"] = float(np.abs(d).max())
+            del d
+        runs[prec] = rec
+        print(f"[precision] {prec} {rec}", file=sys.stderr, flush=True)
+        c.close()
+        del m, j
+    line = {
+        "metric": "velocity L2 error vs FP64 (shifted 16-bit storage)",
+        "value": runs["f16s"]["l2_rel"],
+        "unit": "relative L2",
+        "n_gpus": 1,
+        "higher_is_better": False,
+        "dtype": "f16s",
+        "data": "synthetic",
+        "config": {"workload": f"lid-driven cavity {size[0]}^3, lid {args.u_lid}, Re {re:.1f} "
+                               f"(seed {args.seed}), to residual 1e-6 or {args.steps} steps",
+                   "reynolds": re},
+        "runs": runs,
+        "mlups_ratio_f16s_over_f32": round(runs["f16s"]["mlups"] / runs["f32"]["mlups"], 3),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,6 +231,8 @@ def main():
     ap.add_argument("--launch-alpha", type=int, default=None,
                     help="LBM_LAUNCH_* bits for the alpha kernel (default: library default)")
     ap.add_argument("--launch-beta", type=int, default=None)
+    ap.add_argument("--workload", default="step", choices=("step", "precision"))
+    ap.add_argument("--seed", type=int, default=20261202)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -176,6 +249,9 @@ def main():
     L = _lib.lib()
     device = 0
     stream = hip.stream()
+    if args.workload == "precision":
+        precision_workload(args, cf, L, hip, stream)
+        return
     size = (args.size,) * 3
     params = cf.cavity_parameters(size, args.u_lid, args.reynolds, args.precision)
 

@@ -26,7 +26,7 @@ EXPORTED = (
     "lbm_get_velocities", "lbm_set_velocities", "lbm_get_densities", "lbm_set_densities",
     "lbm_face_box", "lbm_face_elements", "lbm_pack_face", "lbm_unpack_face",
     "lbm_exchange_face", "lbm_device_alloc", "lbm_device_free",
-    "lbm_set_launch_options", "lbm_get_launch_options", "lbm_set_step_pack",
+    "lbm_set_launch_options", "lbm_get_launch_options", "lbm_set_step_pack", "lbm_get_moments",
     "lbm_reduce_box", "lbm_residual", "lbm_sync", "lbm_last_error", "lbm_status_string",
     "lbm_abi_version",
 )
@@ -137,6 +137,7 @@ def lib() -> ctypes.CDLL:
         "lbm_set_launch_options": ([P, I, I], I),
         "lbm_get_launch_options": ([P, I, ctypes.POINTER(I)], I),
         "lbm_set_step_pack": ([P, P, P], I),
+        "lbm_get_moments": ([P, I, i3, i3, P, I, P], I),
         "lbm_reduce_box": ([P, i3, i3, ctypes.POINTER(ctypes.c_double), P], I),
         "lbm_residual": ([P, I, ctypes.POINTER(ctypes.c_double), P], I),
         "lbm_sync": ([P, P], I),

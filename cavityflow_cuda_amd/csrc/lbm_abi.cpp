@@ -490,6 +490,32 @@ int lbm_set_density_distributions_dir(lbm_solver* s, const int origin[3], const 
     return copy_set(s, 0, b, kQ, src, src_on_device, mask, stream);
 }
 
+int lbm_get_moments(lbm_solver* s, int layout, const int origin[3], const int size[3],
+                    void* dst, int dst_on_device, hipStream_t stream)
+{
+    Box b;
+    int rc = resolve_box(s, origin, size, 1, &b);
+    if (rc)
+        return rc;
+    if (!dst || layout < 0 || layout > 1) {
+        set_error("lbm_get_moments: invalid layout or NULL destination");
+        return LBM_ERR_ARG;
+    }
+    if ((rc = set_device(s)))
+        return rc;
+    const size_t bytes = (size_t)b.cells() * 4 * s->out_bytes;
+    if (dst_on_device) {
+        LBM_HIP(launch_moments(s, b, layout, dst, stream), "moments kernel");
+        return LBM_OK;
+    }
+    if ((rc = ensure_staging(s, bytes)))
+        return rc;
+    LBM_HIP(launch_moments(s, b, layout, s->staging, stream), "moments kernel");
+    LBM_HIP(hipMemcpyAsync(dst, s->staging, bytes, hipMemcpyDeviceToHost, stream), "D2H copy");
+    LBM_HIP(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    return LBM_OK;
+}
+
 int lbm_get_flags(lbm_solver* s, const int origin[3], const int size[3], int32_t* dst)
 {
     Box b;

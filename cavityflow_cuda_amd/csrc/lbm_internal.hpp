@@ -107,6 +107,8 @@ hipError_t launch_flags_set(const lbm_solver* s, const Box& b, const int32_t* sr
 
 hipError_t launch_reduce_box(const lbm_solver* s, const Box& b, double* partials,
                              double* out_dev, hipStream_t st);
+hipError_t launch_moments(const lbm_solver* s, const Box& b, int layout, void* dst,
+                          hipStream_t st);
 hipError_t launch_residual(const lbm_solver* s, int layout, double* partials, double* out_dev,
                            hipStream_t st);
 

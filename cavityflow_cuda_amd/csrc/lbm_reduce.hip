@@ -111,6 +111,40 @@ __global__ __launch_bounds__(kRedBlock) void k_residual(const typename P::S* __r
     block_store(acc, partials);
 }
 
+// Density and momentum of every cell of a box, computed from the populations
+// with the reference accumulation order; dst is dense [rho, jx, jy, jz][box].
+template <class P>
+__global__ __launch_bounds__(kRedBlock) void k_moments(const typename P::S* __restrict__ dd,
+                                                       RedArgs a, int layout,
+                                                       typename P::C* __restrict__ dst)
+{
+    using C = typename P::C;
+    const uint32_t gid = blockIdx.x * kRedBlock + threadIdx.x;
+    if (gid >= a.cells)
+        return;
+    const uint32_t t = a.div_bx.div(gid);
+    const uint32_t x = gid - t * a.div_bx.d;
+    const uint32_t z = a.div_by.div(t);
+    const uint32_t y = t - z * a.div_by.d;
+    const uint32_t id = (uint32_t)((a.oz + (int32_t)z) * a.nxy + (a.oy + (int32_t)y) * a.nx +
+                                   a.ox + (int32_t)x);
+    C d[kQ];
+#pragma unroll
+    for (int l = 0; l < kQ; l++) {
+        const C v = P::load(dd[(int64_t)l * a.stride + id], l);
+        if (layout)
+            d[opp(l)] = v;
+        else
+            d[l] = v;
+    }
+    C rho, ux, uy, uz;
+    moments(d, rho, ux, uy, uz);
+    dst[gid] = rho;
+    dst[(int64_t)a.cells + gid] = ux;
+    dst[2 * (int64_t)a.cells + gid] = uy;
+    dst[3 * (int64_t)a.cells + gid] = uz;
+}
+
 __global__ __launch_bounds__(64) void k_finalize(const double* partials, int nblocks,
                                                  double* out)
 {
@@ -152,6 +186,38 @@ static hipError_t residual_t(const lbm_solver* s, int layout, double* partials, 
                        partials);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, st, partials, kReduceBlocks, out);
     return hipGetLastError();
+}
+
+template <class P>
+static hipError_t moments_t(const lbm_solver* s, const Box& b, int layout, void* dst,
+                            hipStream_t st)
+{
+    RedArgs a;
+    a.stride = s->stride;
+    a.nx = s->size_h[0];
+    a.nxy = s->size_h[0] * s->size_h[1];
+    a.ox = b.ox;
+    a.oy = b.oy;
+    a.oz = b.oz;
+    a.cells = b.cells();
+    a.div_bx = FastDiv((uint32_t)b.bx);
+    a.div_by = FastDiv((uint32_t)b.by);
+    hipLaunchKernelGGL(k_moments<P>, dim3((a.cells + kRedBlock - 1) / kRedBlock),
+                       dim3(kRedBlock), 0, st, (const typename P::S*)s->dd, a, layout,
+                       (typename P::C*)dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_moments(const lbm_solver* s, const Box& b, int layout, void* dst,
+                          hipStream_t st)
+{
+    switch (s->p.precision) {
+    case LBM_F32: return moments_t<StoreF32>(s, b, layout, dst, st);
+    case LBM_F64: return moments_t<StoreF64>(s, b, layout, dst, st);
+    case LBM_F16S: return moments_t<StoreF16S>(s, b, layout, dst, st);
+    case LBM_BF16S: return moments_t<StoreBF16S>(s, b, layout, dst, st);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_reduce_box(const lbm_solver* s, const Box& b, double* partials,

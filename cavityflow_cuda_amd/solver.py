@@ -206,6 +206,16 @@ class LbmSolverHIP:
         check(lib().lbm_set_densities(self._h, _i3(origin), _i3(size), ptr, dev,
                                       _stream(stream)))
 
+    def getMoments(self, layout: int, origin=None, size=None, dst=None, stream=None):
+        """[rho, jx, jy, jz] per cell computed from the populations
+        (layout 0: after beta/init, 1: after alpha)."""
+        if dst is None:
+            dst = np.empty(4 * self._cells(origin, size), dtype=self.dtype)
+        ptr, dev, keep = _buffer(dst, self.dtype, writable=True)
+        check(lib().lbm_get_moments(self._h, int(layout), _i3(origin), _i3(size), ptr, dev,
+                                    _stream(stream)))
+        return dst
+
     # -- halo faces (CComm shapes, CManager.cpp:135-202) ---------------------
     def face_box(self, direction, which):
         o = (ctypes.c_int * 3)()

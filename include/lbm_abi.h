@@ -212,6 +212,16 @@ int lbm_get_densities(lbm_solver* solver, const int origin[3], const int size[3]
 int lbm_set_densities(lbm_solver* solver, const int origin[3], const int size[3],
                       const void* src, int src_on_device, hipStream_t stream);
 
+/* Density and momentum computed from the current populations (no stored
+ * outputs needed), in the reference accumulation order (CLbmAlphaCPU.cpp:87-182):
+ * dst is dense [rho, jx, jy, jz][z][y][x] over the box (NULL origin: interior).
+ * layout 0: populations in natural slots (after beta or init);
+ * layout 1: swapped slots (after alpha). The velocity of the reference's
+ * incompressible form is j itself. Complements getVelocities/getDensities,
+ * which return what the last step stored. */
+int lbm_get_moments(lbm_solver* solver, int layout, const int origin[3], const int size[3],
+                    void* dst, int dst_on_device, hipStream_t stream);
+
 /* ---- halo exchange faces (CComm shapes, CManager.cpp:135-202) ------------ */
 
 /* Face boxes of the reference's CComm descriptors for face `direction`:
