@@ -76,6 +76,18 @@ class Hip:
         self.ok(self.L.hipStreamSynchronize(s), "hipStreamSynchronize")
 
 
+def measured_traffic(kernel, precision, size):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE /
+    WRITE_SIZE profile of this workload (profiles/step_f32_512_hbm_traffic.md),
+    or None when no profile covers it."""
+    path = os.path.join(ROOT, "profiles", "step_f32_512_hbm_traffic.md")
+    if precision != "f32" or size != 512 or not os.path.exists(path):
+        return None
+    import re
+    m = re.search(rf"- {kernel}: ([0-9.]+) GB per launch", open(path).read())
+    return int(float(m.group(1)) * 1e9) if m else None
+
+
 def copy_bandwidth(hip, lib_, device, stream, gib=4.0, reps=10):
     """Device-to-device copy rate (read + write bytes / time) for context."""
     nbytes = int(gib * 2**30)
@@ -394,7 +406,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(f"k_{dom}", args.precision, args.size),
             "bytes_per_launch": alg_bytes,
             "mean_ms": {k: round(v, 4) for k, v in kern.items()},
         },

@@ -33,6 +33,22 @@ def hipcc() -> str:
     return exe
 
 
+DRIVER_SRC = os.path.join(CSRC, "host", "cavity_driver.cpp")
+DRIVER_PATH = os.path.join(HERE, "cavity_driver")
+
+
+def build_driver(verbose: bool = False) -> str:
+    """The native C++ cavity driver over the C ABI (plain host code, g++)."""
+    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), DRIVER_SRC,
+           "-o", DRIVER_PATH + ".tmp", "-L", HERE, "-lcavityflow_hip",
+           "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(DRIVER_PATH + ".tmp", DRIVER_PATH)
+    return DRIVER_PATH
+
+
 def build(verbose: bool = False) -> str:
     """Compile every source to an object and link the shared library."""
     objdir = os.path.join(HERE, "_build")
@@ -53,6 +69,7 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB_PATH)
+    build_driver(verbose)
     return LIB_PATH
 
 
