@@ -27,6 +27,9 @@ EXPORTED = (
     "lbm_face_box", "lbm_face_elements", "lbm_pack_face", "lbm_unpack_face",
     "lbm_exchange_face", "lbm_device_alloc", "lbm_device_free",
     "lbm_set_launch_options", "lbm_get_launch_options", "lbm_set_step_pack", "lbm_get_moments",
+    "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_size", "lbm_batch_set_tuning",
+    "lbm_batch_get_tuning", "lbm_batch_step", "lbm_batch_run", "lbm_batch_autotune",
+    "lbm_batch_residual", "lbm_reset",
     "lbm_reduce_box", "lbm_residual", "lbm_sync", "lbm_last_error", "lbm_status_string",
     "lbm_abi_version",
 )
@@ -75,6 +78,15 @@ class lbm_derived(ctypes.Structure):
         ("tau", ctypes.c_double),
         ("inv_tau", ctypes.c_double),
         ("timestep", ctypes.c_double),
+    ]
+
+
+class lbm_batch_tuning(ctypes.Structure):
+    _fields_ = [
+        ("block", ctypes.c_int),
+        ("cells_per_thread", ctypes.c_int),
+        ("waves_per_cu", ctypes.c_int),
+        ("options", ctypes.c_int * 2),
     ]
 
 
@@ -138,6 +150,16 @@ def lib() -> ctypes.CDLL:
         "lbm_get_launch_options": ([P, I, ctypes.POINTER(I)], I),
         "lbm_set_step_pack": ([P, P, P], I),
         "lbm_get_moments": ([P, I, i3, i3, P, I, P], I),
+        "lbm_batch_create": ([ctypes.POINTER(P), I, ctypes.POINTER(P)], I),
+        "lbm_batch_destroy": ([P], I),
+        "lbm_batch_size": ([P], I),
+        "lbm_batch_set_tuning": ([P, ctypes.POINTER(lbm_batch_tuning)], I),
+        "lbm_batch_get_tuning": ([P, ctypes.POINTER(lbm_batch_tuning)], I),
+        "lbm_batch_step": ([P, I, P], I),
+        "lbm_batch_run": ([P, I, I, P], I),
+        "lbm_batch_autotune": ([P, I, P, ctypes.POINTER(lbm_batch_tuning)], I),
+        "lbm_batch_residual": ([P, I, ctypes.POINTER(ctypes.c_double), P], I),
+        "lbm_reset": ([P], I),
         "lbm_reduce_box": ([P, i3, i3, ctypes.POINTER(ctypes.c_double), P], I),
         "lbm_residual": ([P, I, ctypes.POINTER(ctypes.c_double), P], I),
         "lbm_sync": ([P, P], I),

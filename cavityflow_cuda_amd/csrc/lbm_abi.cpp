@@ -345,7 +345,7 @@ int lbm_create(const lbm_params* params, lbm_solver** out)
     if (hipMalloc((void**)&s->partials, (size_t)(kReduceBlocks + 1) * 5 * sizeof(double)) !=
         hipSuccess)
         return fail(LBM_ERR_NOMEM, "cannot allocate reduction buffer");
-    hipError_t e = launch_init(s, nullptr);
+    hipError_t e = launch_init(s, nullptr, 1);
     if (e == hipSuccess)
         e = hipStreamSynchronize(nullptr);
     if (e != hipSuccess) {
@@ -816,6 +816,22 @@ int lbm_residual(lbm_solver* s, int layout, double* residual, hipStream_t stream
         *residual = host[1] > 0 ? std::sqrt(host[0] / host[1])
                                 : (host[0] > 0 ? std::numeric_limits<double>::infinity() : 0.0);
     }
+    return LBM_OK;
+}
+
+int lbm_reset(lbm_solver* s)
+{
+    if (!s) {
+        set_error("lbm_reset: solver is NULL");
+        return LBM_ERR_ARG;
+    }
+    int rc = set_device(s);
+    if (rc)
+        return rc;
+    LBM_HIP(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    LBM_HIP(launch_init(s, nullptr, 0), "init kernel");
+    LBM_HIP(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
+    s->residual_valid = 0;
     return LBM_OK;
 }
 

@@ -1,0 +1,634 @@
+// Many independent cavities stepped together: one launch per half-step covers
+// every cell of every member solver, and runs of half-steps are captured in a
+// HIP graph. Members keep their own arrays (created by lbm_create), so a
+// member's result is bit-identical alone and in a batch: the per-cell code is
+// the same alpha_cell / beta_cell.
+//
+// Work is cut into tiles of block * cells_per_thread consecutive cells of one
+// member; tile_member[t] names the member of tile t. Either one workgroup per
+// tile (optionally renumbered per XCD) or a persistent grid of
+// waves_per_cu * CUs workgroups striding over the tiles. An autotuner times
+// the candidate shapes on the actual batch and keeps the fastest.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "lbm_abi.h"
+#include "lbm_cell.hpp"
+
+namespace lbm {
+
+struct MemberDesc {
+    const char* plane[2][kQ];   // alpha bases, beta bases (shifted by c_l)
+    const uint8_t* flags;
+    void* rho;
+    void* u;
+    void* prev;                 // residual snapshot, 3 planes of the compute type
+    int64_t stride;
+    int32_t n, nx, nxy;
+    uint32_t tile_begin;
+    int32_t store_rho, store_u;
+    double inv_tau, lid, g0, g1, g2;
+};
+
+template <class C>
+__device__ __forceinline__ StepArgs<C> member_args(const MemberDesc& d, int phase)
+{
+    StepArgs<C> a;
+    a.dd = nullptr;
+    a.flags = d.flags;
+    a.rho = (C*)d.rho;
+    a.u = (C*)d.u;
+    a.stride = d.stride;
+    a.n = d.n;
+    a.nx = d.nx;
+    a.nxy = d.nxy;
+    a.inv_tau = (C)d.inv_tau;
+    a.lid = (C)d.lid;
+    a.g0 = (C)d.g0;
+    a.g1 = (C)d.g1;
+    a.g2 = (C)d.g2;
+    a.store_rho = d.store_rho;
+    a.store_u = d.store_u;
+    a.xpack_on = 0;
+    a.xpack[0] = a.xpack[1] = nullptr;
+#pragma unroll
+    for (int l = 0; l < kQ; l++)
+        a.plane[l] = d.plane[phase][l];
+    return a;
+}
+
+// Tile iteration: first tile, stride and count for this workgroup.
+__device__ __forceinline__ void tile_range(uint32_t tiles, int persistent, int xcd,
+                                           uint32_t& first, uint32_t& step)
+{
+    if (!persistent) {
+        first = xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+        step = tiles;   // one tile only
+    } else {
+        first = blockIdx.x;
+        step = gridDim.x;
+    }
+}
+
+template <class P, int BLOCK, bool NT>
+__global__ __launch_bounds__(BLOCK) void k_batch(const MemberDesc* __restrict__ mem,
+                                                 const int32_t* __restrict__ tile_member,
+                                                 uint32_t tiles, int phase, int cpt,
+                                                 int persistent, int xcd)
+{
+    using C = typename P::C;
+    using S = typename P::S;
+    uint32_t first, step;
+    tile_range(tiles, persistent, xcd, first, step);
+    const uint32_t tile_cells = (uint32_t)BLOCK * (uint32_t)cpt;
+    for (uint32_t t = first; t < tiles; t += step) {
+        const int m = tile_member[t];
+        const MemberDesc& d = mem[m];
+        const StepArgs<C> a = member_args<C>(d, phase);
+        const uint32_t base = (t - d.tile_begin) * tile_cells;
+        if (phase == 0) {
+            for (int k = 0; k < cpt; k++) {
+                const uint32_t id = base + (uint32_t)k * BLOCK + threadIdx.x;
+                if (id < (uint32_t)a.n) {
+                    const Acc32<S, NT> acc{a.plane, id * (uint32_t)sizeof(S)};
+                    alpha_cell<P>(a, id, acc);
+                }
+            }
+        } else {
+            const int64_t reach = (int64_t)a.nxy + a.nx + 1;
+            const int64_t last = std::min<int64_t>((int64_t)base + tile_cells, a.n) - 1;
+            const bool wrap = (int64_t)base < reach || last + reach >= (int64_t)a.n;
+            for (int k = 0; k < cpt; k++) {
+                const uint32_t id = base + (uint32_t)k * BLOCK + threadIdx.x;
+                if (id >= (uint32_t)a.n)
+                    continue;
+                if (wrap) {
+                    // the wrap path addresses planes from the unshifted base
+                    const Acc64<S, true> acc{(S*)d.plane[0][0], a.stride, (int32_t)id, a.n};
+                    beta_cell<P>(a, id, acc);
+                } else {
+                    const Acc32<S, false> acc{a.plane, id * (uint32_t)sizeof(S)};
+                    beta_cell<P>(a, id, acc);
+                }
+            }
+        }
+        if (!persistent)
+            break;
+    }
+}
+
+// Per-member residual sums over FLUID cells: out[2m] += |u - u_prev|^2,
+// out[2m + 1] += |u|^2 (FP64, one atomic per workgroup and member tile).
+template <class P>
+__global__ __launch_bounds__(256) void k_batch_residual(const MemberDesc* __restrict__ mem,
+                                                        const int32_t* __restrict__ tile_member,
+                                                        uint32_t tiles, uint32_t tile_cells,
+                                                        int layout, double* out)
+{
+    using C = typename P::C;
+    using S = typename P::S;
+    __shared__ double lds[4][2];
+    for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const int m = tile_member[t];
+        const MemberDesc& d = mem[m];
+        const S* dd = (const S*)d.plane[0][0];
+        C* prev = (C*)d.prev;
+        const uint32_t base = (t - d.tile_begin) * tile_cells;
+        double acc0 = 0.0, acc1 = 0.0;
+        for (uint32_t k = threadIdx.x; k < tile_cells; k += 256) {
+            const uint32_t id = base + k;
+            if (id >= (uint32_t)d.n || d.flags[id] != kFluid)
+                continue;
+            C f[kQ];
+#pragma unroll
+            for (int l = 0; l < kQ; l++) {
+                const C v = P::load(dd[(int64_t)l * d.stride + id], l);
+                if (layout)
+                    f[opp(l)] = v;
+                else
+                    f[l] = v;
+            }
+            C rho, ux, uy, uz;
+            moments(f, rho, ux, uy, uz);
+            const double dx = (double)ux - (double)prev[id];
+            const double dy = (double)uy - (double)prev[d.stride + id];
+            const double dz = (double)uz - (double)prev[2 * d.stride + id];
+            acc0 += dx * dx + dy * dy + dz * dz;
+            acc1 += (double)ux * ux + (double)uy * uy + (double)uz * uz;
+            prev[id] = ux;
+            prev[d.stride + id] = uy;
+            prev[2 * d.stride + id] = uz;
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            acc0 += __shfl_xor(acc0, off, 64);
+            acc1 += __shfl_xor(acc1, off, 64);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            lds[threadIdx.x >> 6][0] = acc0;
+            lds[threadIdx.x >> 6][1] = acc1;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            atomicAdd(&out[2 * m], lds[0][0] + lds[1][0] + lds[2][0] + lds[3][0]);
+            atomicAdd(&out[2 * m + 1], lds[0][1] + lds[1][1] + lds[2][1] + lds[3][1]);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_zero(double* p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = 0.0;
+}
+
+}  // namespace lbm
+
+using namespace lbm;
+
+struct GraphEntry {
+    int first_phase, half_steps;
+    hipStream_t stream;
+    hipGraphExec_t exec;
+};
+
+struct lbm_batch {
+    std::vector<lbm_solver*> members;
+    int precision, device, cus;
+    lbm_batch_tuning tuning;
+    std::vector<MemberDesc> host_desc;
+    MemberDesc* desc = nullptr;
+    int32_t* tile_member = nullptr;
+    uint32_t tiles = 0;        // tiles of the current tuning
+    uint32_t res_tiles = 0;    // tiles of 1024 cells for the residual
+    int32_t* res_member = nullptr;
+    MemberDesc* res_desc = nullptr;
+    double* res_out = nullptr;
+    std::vector<GraphEntry> graphs;
+};
+
+namespace {
+
+#define BATCH_HIP(call, what)                  \
+    do {                                       \
+        hipError_t e_ = (call);                \
+        if (e_ != hipSuccess)                  \
+            return hip_fail(e_, what);         \
+    } while (0)
+
+void drop_graphs(lbm_batch* b)
+{
+    for (auto& g : b->graphs)
+        (void)hipGraphExecDestroy(g.exec);
+    b->graphs.clear();
+}
+
+int upload_tiles(lbm_batch* b, uint32_t tile_cells, std::vector<MemberDesc>& desc,
+                 int32_t** member_dev, uint32_t* ntiles, MemberDesc** desc_dev)
+{
+    std::vector<int32_t> map;
+    for (size_t m = 0; m < b->members.size(); m++) {
+        desc[m].tile_begin = (uint32_t)map.size();
+        const uint32_t t = (uint32_t)((b->members[m]->n + tile_cells - 1) / tile_cells);
+        map.insert(map.end(), t, (int32_t)m);
+    }
+    if (*member_dev)
+        BATCH_HIP(hipFree(*member_dev), "hipFree");
+    if (*desc_dev)
+        BATCH_HIP(hipFree(*desc_dev), "hipFree");
+    *member_dev = nullptr;
+    *desc_dev = nullptr;
+    if (hipMalloc((void**)member_dev, map.size() * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc((void**)desc_dev, desc.size() * sizeof(MemberDesc)) != hipSuccess) {
+        set_error("batch: cannot allocate the tile table");
+        return LBM_ERR_NOMEM;
+    }
+    BATCH_HIP(hipMemcpy(*member_dev, map.data(), map.size() * sizeof(int32_t),
+                        hipMemcpyHostToDevice), "H2D tile table");
+    BATCH_HIP(hipMemcpy(*desc_dev, desc.data(), desc.size() * sizeof(MemberDesc),
+                        hipMemcpyHostToDevice), "H2D member table");
+    *ntiles = (uint32_t)map.size();
+    return LBM_OK;
+}
+
+bool valid_tuning(const lbm_batch_tuning& t)
+{
+    return (t.block == 256 || t.block == 512) &&
+           (t.cells_per_thread == 1 || t.cells_per_thread == 2 || t.cells_per_thread == 4) &&
+           t.waves_per_cu >= 0 && t.waves_per_cu <= 64 &&
+           !(t.options[0] & ~(LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_XCD_REMAP)) &&
+           !(t.options[1] & ~(LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_XCD_REMAP));
+}
+
+int apply_tuning(lbm_batch* b, const lbm_batch_tuning& t)
+{
+    drop_graphs(b);
+    b->tuning = t;
+    return upload_tiles(b, (uint32_t)(t.block * t.cells_per_thread), b->host_desc,
+                        &b->tile_member, &b->tiles, &b->desc);
+}
+
+template <class P, int BLOCK>
+hipError_t launch_phase_t(const lbm_batch* b, int phase, hipStream_t st)
+{
+    const lbm_batch_tuning& t = b->tuning;
+    const int opts = t.options[phase];
+    const int persistent = t.waves_per_cu > 0;
+    const uint32_t grid = persistent ? std::min<uint32_t>(b->tiles, (uint32_t)t.waves_per_cu * b->cus)
+                                     : b->tiles;
+    const int xcd = (opts & LBM_LAUNCH_XCD_REMAP) ? 1 : 0;
+    // nontemporal accesses only for alpha (beta's shifted pattern loses with them)
+    if (phase == 0 && (opts & LBM_LAUNCH_NONTEMPORAL))
+        hipLaunchKernelGGL((k_batch<P, BLOCK, true>), dim3(grid), dim3(BLOCK), 0, st, b->desc,
+                           b->tile_member, b->tiles, phase, t.cells_per_thread, persistent, xcd);
+    else
+        hipLaunchKernelGGL((k_batch<P, BLOCK, false>), dim3(grid), dim3(BLOCK), 0, st, b->desc,
+                           b->tile_member, b->tiles, phase, t.cells_per_thread, persistent, xcd);
+    return hipGetLastError();
+}
+
+template <class P>
+hipError_t launch_phase_p(const lbm_batch* b, int phase, hipStream_t st)
+{
+    return b->tuning.block == 512 ? launch_phase_t<P, 512>(b, phase, st)
+                                  : launch_phase_t<P, 256>(b, phase, st);
+}
+
+hipError_t launch_phase(const lbm_batch* b, int phase, hipStream_t st)
+{
+    switch (b->precision) {
+    case LBM_F32: return launch_phase_p<StoreF32>(b, phase, st);
+    case LBM_F64: return launch_phase_p<StoreF64>(b, phase, st);
+    case LBM_F16S: return launch_phase_p<StoreF16S>(b, phase, st);
+    case LBM_BF16S: return launch_phase_p<StoreBF16S>(b, phase, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+int set_dev(int device)
+{
+    int cur = -1;
+    BATCH_HIP(hipGetDevice(&cur), "hipGetDevice");
+    if (cur != device)
+        BATCH_HIP(hipSetDevice(device), "hipSetDevice");
+    return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_batch_create(lbm_solver* const* members, int count, lbm_batch** out)
+{
+    if (!members || count <= 0 || !out) {
+        set_error("lbm_batch_create: invalid argument");
+        return LBM_ERR_ARG;
+    }
+    *out = nullptr;
+    const int prec = members[0]->p.precision, dev = members[0]->p.device;
+    for (int i = 0; i < count; i++) {
+        const lbm_solver* s = members[i];
+        if (!s || s->p.precision != prec || s->p.device != dev) {
+            set_error("lbm_batch_create: members must share precision and device");
+            return LBM_ERR_ARG;
+        }
+        if ((uint64_t)s->stride * (uint64_t)s->elem_bytes >= (1ull << 32)) {
+            set_error("lbm_batch_create: a member is too large for 32-bit plane offsets");
+            return LBM_ERR_UNSUPPORTED;
+        }
+        for (int j = 0; j < i; j++)
+            if (members[j] == s) {
+                set_error("lbm_batch_create: a solver appears twice");
+                return LBM_ERR_ARG;
+            }
+    }
+    int rc = set_dev(dev);
+    if (rc)
+        return rc;
+    lbm_batch* b = new lbm_batch();
+    b->members.assign(members, members + count);
+    b->precision = prec;
+    b->device = dev;
+    if (hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        b->cus = 256;
+    b->host_desc.resize(count);
+    for (int i = 0; i < count; i++) {
+        lbm_solver* s = members[i];
+        MemberDesc& d = b->host_desc[i];
+        const int nx = s->size_h[0], nxy = s->size_h[0] * s->size_h[1];
+        for (int l = 0; l < kQ; l++) {
+            const int64_t base = (int64_t)l * s->stride * s->elem_bytes;
+            const int delta = cx(l) + cy(l) * nx + cz(l) * nxy;
+            d.plane[0][l] = (const char*)s->dd + base;
+            d.plane[1][l] = (const char*)s->dd + base + (int64_t)delta * s->elem_bytes;
+        }
+        d.flags = s->flags;
+        d.rho = s->rho;
+        d.u = s->u;
+        d.prev = nullptr;
+        d.stride = s->stride;
+        d.n = (int32_t)s->n;
+        d.nx = nx;
+        d.nxy = nxy;
+        d.store_rho = s->p.store_densities;
+        d.store_u = s->p.store_velocities;
+        d.inv_tau = s->p.inv_tau;
+        d.lid = s->p.lid_velocity;
+        d.g0 = s->p.gravitation[0];
+        d.g1 = s->p.gravitation[1];
+        d.g2 = s->p.gravitation[2];
+    }
+    lbm_batch_tuning t;
+    t.block = 256;
+    t.cells_per_thread = 1;
+    t.waves_per_cu = 0;
+    t.options[0] = LBM_LAUNCH_NONTEMPORAL;
+    t.options[1] = LBM_LAUNCH_XCD_REMAP;
+    if ((rc = apply_tuning(b, t))) {
+        lbm_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return LBM_OK;
+}
+
+int lbm_batch_destroy(lbm_batch* b)
+{
+    if (!b)
+        return LBM_OK;
+    (void)set_dev(b->device);
+    (void)hipDeviceSynchronize();
+    drop_graphs(b);
+    void* bufs[] = {b->desc, b->tile_member, b->res_member, b->res_desc, b->res_out};
+    for (void* p : bufs)
+        if (p)
+            (void)hipFree(p);
+    delete b;
+    return LBM_OK;
+}
+
+int lbm_batch_size(const lbm_batch* b) { return b ? (int)b->members.size() : 0; }
+
+int lbm_batch_set_tuning(lbm_batch* b, const lbm_batch_tuning* t)
+{
+    if (!b || !t || !valid_tuning(*t)) {
+        set_error("lbm_batch_set_tuning: invalid tuning");
+        return LBM_ERR_ARG;
+    }
+    int rc = set_dev(b->device);
+    if (rc)
+        return rc;
+    return apply_tuning(b, *t);
+}
+
+int lbm_batch_get_tuning(const lbm_batch* b, lbm_batch_tuning* t)
+{
+    if (!b || !t) {
+        set_error("lbm_batch_get_tuning: invalid argument");
+        return LBM_ERR_ARG;
+    }
+    *t = b->tuning;
+    return LBM_OK;
+}
+
+int lbm_batch_step(lbm_batch* b, int phase, hipStream_t stream)
+{
+    if (!b || phase < 0 || phase > 1) {
+        set_error("lbm_batch_step: invalid argument");
+        return LBM_ERR_ARG;
+    }
+    int rc = set_dev(b->device);
+    if (rc)
+        return rc;
+    BATCH_HIP(launch_phase(b, phase, stream), "batch kernel launch");
+    return LBM_OK;
+}
+
+int lbm_batch_run(lbm_batch* b, int first_phase, int half_steps, hipStream_t stream)
+{
+    if (!b || first_phase < 0 || first_phase > 1 || half_steps < 0) {
+        set_error("lbm_batch_run: invalid argument");
+        return LBM_ERR_ARG;
+    }
+    int rc = set_dev(b->device);
+    if (rc)
+        return rc;
+    if (half_steps == 0)
+        return LBM_OK;
+    if (!stream) {   // the legacy default stream cannot be captured
+        for (int i = 0; i < half_steps; i++)
+            BATCH_HIP(launch_phase(b, (first_phase + i) & 1, nullptr), "batch kernel launch");
+        return LBM_OK;
+    }
+    for (auto& g : b->graphs)
+        if (g.first_phase == first_phase && g.half_steps == half_steps && g.stream == stream) {
+            BATCH_HIP(hipGraphLaunch(g.exec, stream), "hipGraphLaunch");
+            return LBM_OK;
+        }
+    hipGraph_t graph;
+    BATCH_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+    hipError_t le = hipSuccess;
+    for (int i = 0; i < half_steps && le == hipSuccess; i++)
+        le = launch_phase(b, (first_phase + i) & 1, stream);
+    hipError_t ee = hipStreamEndCapture(stream, &graph);
+    if (le != hipSuccess)
+        return hip_fail(le, "batch kernel capture");
+    if (ee != hipSuccess)
+        return hip_fail(ee, "hipStreamEndCapture");
+    GraphEntry g{first_phase, half_steps, stream, nullptr};
+    hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess)
+        return hip_fail(ie, "hipGraphInstantiate");
+    b->graphs.push_back(g);
+    BATCH_HIP(hipGraphLaunch(g.exec, stream), "hipGraphLaunch");
+    return LBM_OK;
+}
+
+int lbm_batch_autotune(lbm_batch* b, int trial_steps, hipStream_t stream,
+                       lbm_batch_tuning* chosen)
+{
+    if (!b || trial_steps < 2) {
+        set_error("lbm_batch_autotune: invalid argument");
+        return LBM_ERR_ARG;
+    }
+    int rc = set_dev(b->device);
+    if (rc)
+        return rc;
+    // Each phase is tuned on its own; the members' state advances by whole
+    // alpha+beta steps during the trials, exactly as in a run.
+    hipEvent_t e0, e1;
+    BATCH_HIP(hipEventCreate(&e0), "hipEventCreate");
+    BATCH_HIP(hipEventCreate(&e1), "hipEventCreate");
+    const int blocks[] = {256, 512};
+    const int cpts[] = {1, 2, 4};
+    const int wpcs[] = {0, 4, 8, 16};
+    lbm_batch_tuning best = b->tuning;
+    float best_ms[2] = {std::numeric_limits<float>::infinity(), std::numeric_limits<float>::infinity()};
+    lbm_batch_tuning cand;
+    for (int blk : blocks)
+        for (int cpt : cpts)
+            for (int wpc : wpcs)
+                for (int opt = 0; opt < 4; opt++) {
+                    cand.block = blk;
+                    cand.cells_per_thread = cpt;
+                    cand.waves_per_cu = wpc;
+                    cand.options[0] = cand.options[1] = opt;
+                    if ((rc = apply_tuning(b, cand)))
+                        return rc;
+                    float ms[2] = {0.f, 0.f};
+                    for (int i = 0; i < trial_steps; i++)
+                        for (int ph = 0; ph < 2; ph++) {
+                            BATCH_HIP(hipEventRecord(e0, stream), "hipEventRecord");
+                            BATCH_HIP(launch_phase(b, ph, stream), "batch kernel launch");
+                            BATCH_HIP(hipEventRecord(e1, stream), "hipEventRecord");
+                            BATCH_HIP(hipEventSynchronize(e1), "hipEventSynchronize");
+                            float t = 0.f;
+                            BATCH_HIP(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
+                            if (i > 0)
+                                ms[ph] += t;
+                        }
+                    for (int ph = 0; ph < 2; ph++)
+                        if (ms[ph] < best_ms[ph]) {
+                            best_ms[ph] = ms[ph];
+                            if (ph == 0) {
+                                best.block = blk;
+                                best.cells_per_thread = cpt;
+                                best.waves_per_cu = wpc;
+                            }
+                            best.options[ph] = opt;
+                        }
+                }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    // block / cells / grid shape are shared by both phases: keep alpha's, which
+    // carries more arithmetic; beta keeps its own launch options.
+    if ((rc = apply_tuning(b, best)))
+        return rc;
+    if (chosen)
+        *chosen = best;
+    return LBM_OK;
+}
+
+int lbm_batch_residual(lbm_batch* b, int layout, double* residuals, hipStream_t stream)
+{
+    if (!b || !residuals || layout < 0 || layout > 1) {
+        set_error("lbm_batch_residual: invalid argument");
+        return LBM_ERR_ARG;
+    }
+    int rc = set_dev(b->device);
+    if (rc)
+        return rc;
+    const int count = (int)b->members.size();
+    if (!b->res_out) {
+        for (int i = 0; i < count; i++) {
+            lbm_solver* s = b->members[i];
+            if (!s->residual_prev) {
+                const size_t bytes = (size_t)3 * s->stride * s->out_bytes;
+                if (hipMalloc(&s->residual_prev, bytes) != hipSuccess) {
+                    set_error("batch residual: out of device memory");
+                    return LBM_ERR_NOMEM;
+                }
+                BATCH_HIP(hipMemset(s->residual_prev, 0, bytes), "hipMemset");
+                s->residual_valid = 0;
+            }
+            b->host_desc[i].prev = s->residual_prev;
+        }
+        std::vector<MemberDesc> desc = b->host_desc;
+        if ((rc = upload_tiles(b, 1024, desc, &b->res_member, &b->res_tiles, &b->res_desc)))
+            return rc;
+        if (hipMalloc((void**)&b->res_out, (size_t)2 * count * sizeof(double)) != hipSuccess) {
+            set_error("batch residual: out of device memory");
+            return LBM_ERR_NOMEM;
+        }
+    }
+    hipLaunchKernelGGL(k_zero, dim3((2 * count + 255) / 256), dim3(256), 0, stream, b->res_out,
+                       2 * count);
+    const uint32_t grid = std::min<uint32_t>(b->res_tiles, 8u * b->cus);
+    switch (b->precision) {
+    case LBM_F32:
+        hipLaunchKernelGGL(k_batch_residual<StoreF32>, dim3(grid), dim3(256), 0, stream,
+                           b->res_desc, b->res_member, b->res_tiles, 1024u, layout, b->res_out);
+        break;
+    case LBM_F64:
+        hipLaunchKernelGGL(k_batch_residual<StoreF64>, dim3(grid), dim3(256), 0, stream,
+                           b->res_desc, b->res_member, b->res_tiles, 1024u, layout, b->res_out);
+        break;
+    case LBM_F16S:
+        hipLaunchKernelGGL(k_batch_residual<StoreF16S>, dim3(grid), dim3(256), 0, stream,
+                           b->res_desc, b->res_member, b->res_tiles, 1024u, layout, b->res_out);
+        break;
+    default:
+        hipLaunchKernelGGL(k_batch_residual<StoreBF16S>, dim3(grid), dim3(256), 0, stream,
+                           b->res_desc, b->res_member, b->res_tiles, 1024u, layout, b->res_out);
+        break;
+    }
+    BATCH_HIP(hipGetLastError(), "batch residual launch");
+    std::vector<double> host(2 * count);
+    BATCH_HIP(hipMemcpyAsync(host.data(), b->res_out, host.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, stream), "D2H copy");
+    BATCH_HIP(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    // the snapshot and its validity are the member's own, shared with
+    // lbm_residual and cleared by lbm_reset
+    for (int i = 0; i < count; i++) {
+        lbm_solver* s = b->members[i];
+        if (!s->residual_valid) {
+            residuals[i] = std::numeric_limits<double>::infinity();
+            s->residual_valid = 1;
+        } else {
+            const double num = host[2 * i], den = host[2 * i + 1];
+            residuals[i] = den > 0 ? std::sqrt(num / den)
+                                   : (num > 0 ? std::numeric_limits<double>::infinity() : 0.0);
+        }
+    }
+    return LBM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,432 @@
+// Per-cell device code of the A-A step shared by the single-domain kernels
+// (lbm_step.hip) and the multi-cavity batch kernels (lbm_batch.hip):
+// population accessors, one cell of alpha and of beta, fused x-face packing.
+#pragma once
+
+#include "lbm_d3q19.hpp"
+#include "lbm_internal.hpp"
+#include "lbm_storage.hpp"
+
+namespace lbm {
+
+
+constexpr int kBlock = 256;
+
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2). Renumber them so every XCD works on one contiguous range of cells:
+// neighbouring cells then share partial cache lines inside one L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb)
+{
+    const uint32_t q = nb >> 3, r = nb & 7u, x = b & 7u, k = b >> 3;
+    return x * q + (x < r ? x : r) + k;
+}
+
+template <class C>
+__device__ __forceinline__ uint32_t box_cell(const StepArgs<C>& a, uint32_t gid)
+{
+    const uint32_t t = a.div_bx.div(gid);
+    const uint32_t x = gid - t * a.div_bx.d;
+    const uint32_t z = a.div_by.div(t);
+    const uint32_t y = t - z * a.div_by.d;
+    return (uint32_t)((a.oz + (int32_t)z) * a.nxy + (a.oy + (int32_t)y) * a.nx + a.ox + (int32_t)x);
+}
+
+// Population accessors. Acc32 keeps one uniform base pointer per plane (the
+// beta neighbour shift is folded into it) and one 32-bit byte offset per lane
+// shared by all 19 planes, so every access is a saddr global load/store with
+// no per-plane vector address arithmetic; it needs cells * sizeof(S) < 2^32.
+// Acc64 uses a 64-bit element index per plane and, with WRAP, the reference's
+// linear-index wrap (domainWrap, CLbmBetaCPU.cpp:44-50).
+// The plane bases come precomputed in the kernel arguments: opaque uniform
+// values the compiler cannot fold into per-lane pointer chains.
+template <class S, bool NT>
+struct Acc32 {
+    const char* const* plane;
+    uint32_t off;
+    __device__ __forceinline__ S load(int l, int32_t) const
+    {
+        const S* p = (const S*)(plane[l] + off);
+        if (NT)
+            return __builtin_nontemporal_load(p);
+        return *p;
+    }
+    __device__ __forceinline__ void store(int l, int32_t, S v) const
+    {
+        S* p = (S*)(const_cast<char*>(plane[l]) + off);
+        if (NT)
+            __builtin_nontemporal_store(v, p);
+        else
+            *p = v;
+    }
+};
+
+template <class S, bool WRAP>
+struct Acc64 {
+    S* dd;
+    int64_t stride;
+    int32_t id, n;
+    __device__ __forceinline__ int64_t index(int l, int32_t delta) const
+    {
+        int32_t j = id + delta;
+        if (WRAP) {
+            j = (j >= n) ? j - n : j;
+            j = (j < 0) ? j + n : j;
+        }
+        return (int64_t)l * stride + j;
+    }
+    __device__ __forceinline__ S load(int l, int32_t delta) const { return dd[index(l, delta)]; }
+    __device__ __forceinline__ void store(int l, int32_t delta, S v) const { dd[index(l, delta)] = v; }
+};
+
+// ------------------------------------------------------- fused x packing ---
+
+// Whether the cell lies in the LEFT (x = 1) / RIGHT (x = nx - 2) send plane;
+// row = z * ny + y is its index in the dense face buffers.
+template <class C>
+__device__ __forceinline__ bool xpack_where(const StepArgs<C>& a, uint32_t id, uint32_t& row,
+                                            bool& pl, bool& pr)
+{
+    pl = pr = false;
+    row = 0;
+    if (!a.xpack_on)
+        return false;
+    row = a.div_nx.div(id);
+    const int x = (int)(id - row * (uint32_t)a.nx);
+    pl = x == 1 && a.xpack[0] != nullptr;
+    pr = x == a.nx - 2 && a.xpack[1] != nullptr;
+    return pl || pr;
+}
+
+// After alpha: the 19 slot values of a send-plane cell, in the layout
+// lbm_pack_face writes ([slot][row]); values pass through the storage type so
+// 16-bit storage packs exactly what was stored.
+template <class P>
+__device__ __forceinline__ void xpack_alpha(const StepArgs<typename P::C>& a, uint32_t row,
+                                            bool pl, bool pr, const typename P::C (&v)[kQ])
+{
+    using C = typename P::C;
+#pragma unroll
+    for (int s = 0; s < kQ; s++) {
+        const C w = P::load(P::store(v[s], s), s);
+        if (pl)
+            a.xpack[0][(int64_t)s * a.fcells + row] = w;
+        if (pr)
+            a.xpack[1][(int64_t)s * a.fcells + row] = w;
+    }
+}
+
+// After beta: the populations that cross the x faces. From a LEFT send-plane
+// cell the directions with c_x = -1 land in the LEFT halo plane, from a RIGHT
+// one those with c_x = +1 land in the RIGHT halo plane; each is written at
+// its landing cell's row (reference linear wrap), i.e. exactly the entries
+// of the halo-plane pack that the neighbour's direction-filtered unpack reads.
+template <class P>
+__device__ __forceinline__ void xpack_beta(const StepArgs<typename P::C>& a, uint32_t row,
+                                           bool pl, bool pr, const typename P::C (&v)[kQ])
+{
+    using C = typename P::C;
+#pragma unroll
+    for (int l = 0; l < kQ - 1; l++) {
+        const bool side = cx(l) < 0 ? pl : (cx(l) > 0 ? pr : false);
+        if (cx(l) != 0 && side) {
+            int32_t t = (int32_t)row + cy(l) + cz(l) * a.ny;
+            t = t < 0 ? t + a.fcells : t;
+            t = t >= a.fcells ? t - a.fcells : t;
+            a.xpack[cx(l) < 0 ? 0 : 1][(int64_t)l * a.fcells + t] = P::load(P::store(v[l], l), l);
+        }
+    }
+}
+
+// ----------------------------------------------------------------- alpha ---
+
+// One cell of CLbmAlphaCPU::alphaKernelCPU (CLbmAlphaCPU.cpp:75-494). The
+// reference's `tmp` is a #define for `rho` (:199): every gravity term
+// overwrites rho and the stored density is the last gravity term.
+template <class P, class A>
+__device__ __forceinline__ void alpha_cell(const StepArgs<typename P::C>& a, uint32_t id,
+                                           const A& acc)
+{
+    using C = typename P::C;
+    const int flag = a.flags[id];
+    uint32_t row;
+    bool pl, pr;
+    const bool pack = xpack_where(a, id, row, pl, pr);
+    // GHOST_LAYER is skipped (:75-76); OBSTACLE without outputs does nothing
+    if (!pack && (flag == kGhostLayer || (flag == kObstacle && !a.store_u && !a.store_rho)))
+        return;
+
+    C d[kQ];
+#pragma unroll
+    for (int l = 0; l < kQ; l++)
+        d[l] = P::load(acc.load(l, 0), l);
+    if (flag == kGhostLayer) {   // reached only to pack its unchanged slots
+        xpack_alpha<P>(a, row, pl, pr, d);
+        return;
+    }
+    C rho, ux, uy, uz;
+    moments(d, rho, ux, uy, uz);
+
+    if (flag == kFluid || flag == kVelocityInjection) {
+        const C inv_tau = a.inv_tau;
+        const C g0 = a.g0, g1 = a.g1, g2 = a.g2;
+        const bool fluid = (flag == kFluid);
+        C vel2, vela2, vab, dd_param;
+        C e[kQ];
+        if (!fluid) {
+            ux = a.lid;
+            uy = (C)0;
+            uz = (C)0;
+            rho = (C)1;
+        }
+        vel2 = ux * ux + uy * uy + uz * uz;
+        dd_param = rho - ((C)3 / (C)2) * vel2;
+        // post-collision value of direction l: relaxation for FLUID (d += inv_tau*(eq - d)),
+        // the equilibrium itself for VELOCITY_INJECTION
+#define LBM_RELAX(l, eqv) e[l] = fluid ? (d[l] + inv_tau * ((eqv) - d[l])) : (eqv)
+        rho = g0 * ((C)1 / (C)18) * rho;
+        vela2 = ux * ux;
+        LBM_RELAX(1, eq_a1(ux, vela2, dd_param));
+        e[1] -= rho;
+        LBM_RELAX(0, eq_a0(ux, vela2, dd_param));
+        e[0] += rho;
+
+        rho = g1 * ((C)-1 / (C)18) * rho;
+        vela2 = uy * uy;
+        LBM_RELAX(3, eq_a1(uy, vela2, dd_param));
+        e[3] -= rho;
+        LBM_RELAX(2, eq_a0(uy, vela2, dd_param));
+        e[2] += rho;
+
+        vab = ux + uy;
+        vela2 = vab * vab;
+        rho = (g0 - g1) * ((C)1 / (C)36) * rho;
+        LBM_RELAX(5, eq_5(vab, vela2, dd_param));
+        e[5] -= rho;
+        LBM_RELAX(4, eq_4(vab, vela2, dd_param));
+        e[4] += rho;
+
+        vab = ux - uy;
+        vela2 = vab * vab;
+        rho = (g0 + g1) * ((C)1 / (C)36) * rho;
+        LBM_RELAX(7, eq_5(vab, vela2, dd_param));
+        e[7] -= rho;
+        LBM_RELAX(6, eq_4(vab, vela2, dd_param));
+        e[6] += rho;
+
+        vab = ux + uz;
+        vela2 = vab * vab;
+        rho = (g0 + g2) * ((C)1 / (C)36) * rho;
+        LBM_RELAX(9, eq_5(vab, vela2, dd_param));
+        e[9] -= rho;
+        LBM_RELAX(8, eq_4(vab, vela2, dd_param));
+        e[8] += rho;
+
+        rho = (g0 - g2) * ((C)1 / (C)36) * rho;
+        vab = ux - uz;
+        vela2 = vab * vab;
+        LBM_RELAX(11, eq_5(vab, vela2, dd_param));
+        e[11] -= rho;
+        LBM_RELAX(10, eq_4(vab, vela2, dd_param));
+        e[10] += rho;
+
+        vab = uy + uz;
+        vela2 = vab * vab;
+        rho = (g2 - g1) * ((C)1 / (C)36) * rho;
+        LBM_RELAX(13, eq_5(vab, vela2, dd_param));
+        e[13] -= rho;
+        LBM_RELAX(12, eq_4(vab, vela2, dd_param));
+        e[12] += rho;
+
+        vab = uy - uz;
+        vela2 = vab * vab;
+        rho = (g2 + g1) * ((C)-1 / (C)36) * rho;
+        LBM_RELAX(15, eq_5(vab, vela2, dd_param));
+        e[15] -= rho;
+        LBM_RELAX(14, eq_4(vab, vela2, dd_param));
+        e[14] += rho;
+
+        vela2 = uz * uz;
+        rho = g2 * ((C)1 / (C)18) * rho;
+        LBM_RELAX(17, eq_a1(uz, vela2, dd_param));
+        e[17] -= rho;
+        LBM_RELAX(16, eq_a0(uz, vela2, dd_param));
+        e[16] += rho;
+
+        LBM_RELAX(18, eq_18(dd_param));
+#undef LBM_RELAX
+        // direction l goes to slot opp(l) of this cell
+#pragma unroll
+        for (int l = 0; l < kQ; l++)
+            acc.store(opp(l), 0, P::store(e[l], opp(l)));
+        if (pack) {
+            C out[kQ];
+#pragma unroll
+            for (int s2 = 0; s2 < kQ; s2++)
+                out[s2] = e[opp(s2)];
+            xpack_alpha<P>(a, row, pl, pr, out);
+        }
+    } else if (flag == kObstacle) {
+        // no stores: the A-A layout makes bounce-back implicit (:330-343)
+        if (pack)
+            xpack_alpha<P>(a, row, pl, pr, d);
+        if (a.store_u) {
+            ux = (C)0;
+            uy = (C)0;
+            uz = (C)0;
+        }
+    }
+
+    if (a.store_u) {
+        a.u[id] = ux;
+        a.u[a.stride + id] = uy;
+        a.u[2 * a.stride + id] = uz;
+    }
+    if (a.store_rho)
+        a.rho[id] = rho;
+}
+
+
+// ------------------------------------------------------------------ beta ---
+
+// One cell of CLbmBetaCPU::betaKernelCPU (CLbmBetaCPU.cpp:83-451).
+// `dd_param` is a #define for `rho` (:224): the stored FLUID density is
+// rho - 1.5 |j|^2. OBSTACLE swaps opposite pairs, which writes back exactly
+// what was read, so its stores are skipped. GHOST_LAYER passes values through.
+// Neighbour indices wrap modulo the cell count (domainWrap, :44-50).
+template <class P, class A>
+__device__ __forceinline__ void beta_cell(const StepArgs<typename P::C>& a, uint32_t id,
+                                          const A& acc)
+{
+    using C = typename P::C;
+    const int flag = a.flags[id];
+    uint32_t row;
+    bool pl, pr;
+    const bool pack = xpack_where(a, id, row, pl, pr);
+    // OBSTACLE writes back what it reads; without outputs it does nothing
+    if (!pack && flag == kObstacle && !a.store_u && !a.store_rho)
+        return;
+
+    // plane l at id + c_l holds direction opp(l) arriving at this cell
+    C d[kQ];
+#pragma unroll
+    for (int l = 0; l < kQ; l++)
+        d[opp(l)] = P::load(acc.load(l, cx(l) + cy(l) * a.nx + cz(l) * a.nxy), l);
+
+    C rho, ux, uy, uz;
+    moments(d, rho, ux, uy, uz);
+
+    if (flag == kFluid) {
+        const C inv_tau = a.inv_tau;
+        C vel2 = ux * ux + uy * uy + uz * uz;
+        rho = rho - ((C)3 / (C)2) * (vel2);
+        C vela2 = ux * ux;
+        d[0] += inv_tau * (eq_a0(ux, vela2, rho) - d[0]);
+        d[1] += inv_tau * (eq_a1(ux, vela2, rho) - d[1]);
+        vela2 = uy * uy;
+        d[2] += inv_tau * (eq_a0(uy, vela2, rho) - d[2]);
+        d[3] += inv_tau * (eq_a1(uy, vela2, rho) - d[3]);
+        vel2 = ux + uy;
+        vela2 = vel2 * vel2;
+        d[4] += inv_tau * (eq_4(vel2, vela2, rho) - d[4]);
+        d[5] += inv_tau * (eq_5(vel2, vela2, rho) - d[5]);
+        vel2 = ux - uy;
+        vela2 = vel2 * vel2;
+        d[6] += inv_tau * (eq_4(vel2, vela2, rho) - d[6]);
+        d[7] += inv_tau * (eq_5(vel2, vela2, rho) - d[7]);
+        vel2 = ux + uz;
+        vela2 = vel2 * vel2;
+        d[8] += inv_tau * (eq_4(vel2, vela2, rho) - d[8]);
+        d[9] += inv_tau * (eq_5(vel2, vela2, rho) - d[9]);
+        vel2 = ux - uz;
+        vela2 = vel2 * vel2;
+        d[10] += inv_tau * (eq_4(vel2, vela2, rho) - d[10]);
+        d[11] += inv_tau * (eq_5(vel2, vela2, rho) - d[11]);
+        vel2 = uy + uz;
+        vela2 = vel2 * vel2;
+        d[12] += inv_tau * (eq_4(vel2, vela2, rho) - d[12]);
+        d[13] += inv_tau * (eq_5(vel2, vela2, rho) - d[13]);
+        vel2 = uy - uz;
+        vela2 = vel2 * vel2;
+        d[14] += inv_tau * (eq_4(vel2, vela2, rho) - d[14]);
+        d[15] += inv_tau * (eq_5(vel2, vela2, rho) - d[15]);
+        vela2 = uz * uz;
+        d[16] += inv_tau * (eq_a0(uz, vela2, rho) - d[16]);
+        d[17] += inv_tau * (eq_a1(uz, vela2, rho) - d[17]);
+        d[18] += inv_tau * (eq_18(rho) - d[18]);
+    } else if (flag == kVelocityInjection) {
+        ux = a.lid;
+        uy = (C)0;
+        uz = (C)0;
+        rho = (C)1;
+        C vel2 = ux * ux + uy * uy + uz * uz;
+        rho = rho - ((C)3 / (C)2) * vel2;
+        C vela2 = ux * ux;
+        d[0] = eq_a0(ux, vela2, rho);
+        d[1] = eq_a1(ux, vela2, rho);
+        vela2 = uy * uy;
+        d[2] = eq_a0(uy, vela2, rho);
+        d[3] = eq_a1(uy, vela2, rho);
+        vel2 = ux + uy;
+        vela2 = vel2 * vel2;
+        d[4] = eq_4(vel2, vela2, rho);
+        d[5] = eq_5(vel2, vela2, rho);
+        vel2 = ux - uy;
+        vela2 = vel2 * vel2;
+        d[6] = eq_4(vel2, vela2, rho);
+        d[7] = eq_5(vel2, vela2, rho);
+        vel2 = ux + uz;
+        vela2 = vel2 * vel2;
+        d[8] = eq_4(vel2, vela2, rho);
+        d[9] = eq_5(vel2, vela2, rho);
+        vel2 = ux - uz;
+        vela2 = vel2 * vel2;
+        d[10] = eq_4(vel2, vela2, rho);
+        d[11] = eq_5(vel2, vela2, rho);
+        vel2 = uy + uz;
+        vela2 = vel2 * vel2;
+        d[12] = eq_4(vel2, vela2, rho);
+        d[13] = eq_5(vel2, vela2, rho);
+        vel2 = uy - uz;
+        vela2 = vel2 * vel2;
+        d[14] = eq_4(vel2, vela2, rho);
+        d[15] = eq_5(vel2, vela2, rho);
+        vela2 = uz * uz;
+        d[16] = eq_a0(uz, vela2, rho);
+        d[17] = eq_a1(uz, vela2, rho);
+        d[18] = eq_18(rho);
+    } else if (flag == kObstacle) {
+        if (pack) {
+            // the swap writes back the value read from plane l (:302-311)
+            C out[kQ];
+#pragma unroll
+            for (int l = 0; l < kQ; l++)
+                out[l] = d[opp(l)];
+            xpack_beta<P>(a, row, pl, pr, out);
+        }
+        if (a.store_u) {
+            ux = (C)0;
+            uy = (C)0;
+            uz = (C)0;
+        }
+    }
+
+    if (flag != kObstacle) {
+        // direction l goes to slot l of neighbour id + c_l
+#pragma unroll
+        for (int l = 0; l < kQ; l++)
+            acc.store(l, cx(l) + cy(l) * a.nx + cz(l) * a.nxy, P::store(d[l], l));
+        if (pack)
+            xpack_beta<P>(a, row, pl, pr, d);
+    }
+
+    if (a.store_u) {
+        a.u[id] = ux;
+        a.u[a.stride + id] = uy;
+        a.u[2 * a.stride + id] = uz;
+    }
+    if (a.store_rho)
+        a.rho[id] = rho;
+}
+
+
+}  // namespace lbm

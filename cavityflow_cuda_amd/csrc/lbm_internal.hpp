@@ -92,7 +92,7 @@ namespace lbm {
 void set_error(const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
 
-hipError_t launch_init(const lbm_solver* s, hipStream_t st);
+hipError_t launch_init(const lbm_solver* s, hipStream_t st, int write_flags);
 hipError_t launch_alpha(const lbm_solver* s, const Box& b, hipStream_t st);
 hipError_t launch_beta(const lbm_solver* s, const Box& b, hipStream_t st);
 

@@ -291,6 +291,45 @@ int lbm_reduce_box(lbm_solver* solver, const int origin[3], const int size[3], d
  * The first call returns +inf. Synchronous. */
 int lbm_residual(lbm_solver* solver, int layout, double* residual, hipStream_t stream);
 
+/* ---- batched cavities ----------------------------------------------------
+ * No reference counterpart. A batch steps many independent solvers (same
+ * device and precision) with one launch per half-step over all their cells;
+ * each member's result is bit-identical to stepping it alone. Members must
+ * outlive the batch and must all be at the same step parity. */
+typedef struct lbm_batch lbm_batch;
+
+typedef struct lbm_batch_tuning {
+    int block;             /* threads per workgroup: 256 or 512 */
+    int cells_per_thread;  /* 1, 2 or 4 (tile = block * cells_per_thread cells) */
+    int waves_per_cu;      /* 0: one workgroup per tile; k > 0: persistent grid of
+                              k workgroups per CU striding over the tiles */
+    int options[2];        /* LBM_LAUNCH_* for the alpha and beta launches */
+} lbm_batch_tuning;
+
+int lbm_batch_create(lbm_solver* const* members, int count, lbm_batch** out);
+int lbm_batch_destroy(lbm_batch* batch);
+int lbm_batch_size(const lbm_batch* batch);
+int lbm_batch_set_tuning(lbm_batch* batch, const lbm_batch_tuning* tuning);
+int lbm_batch_get_tuning(const lbm_batch* batch, lbm_batch_tuning* tuning);
+/* One half-step (phase 0 alpha, 1 beta) of every member. */
+int lbm_batch_step(lbm_batch* batch, int phase, hipStream_t stream);
+/* half_steps alternating half-steps starting with first_phase. On a non-NULL
+ * stream the sequence is captured once into a HIP graph (per first_phase,
+ * half_steps and stream) and replayed on later calls. */
+int lbm_batch_run(lbm_batch* batch, int first_phase, int half_steps, hipStream_t stream);
+/* Times every candidate shape (block, cells_per_thread, waves_per_cu,
+ * launch options) for trial_steps whole steps each and keeps the fastest.
+ * The trials are real steps: the members advance; reset them afterwards
+ * (lbm_reset) when their state matters. Synchronous. */
+int lbm_batch_autotune(lbm_batch* batch, int trial_steps, hipStream_t stream,
+                       lbm_batch_tuning* chosen);
+/* lbm_residual for every member in one launch; residuals[count]. */
+int lbm_batch_residual(lbm_batch* batch, int layout, double* residuals, hipStream_t stream);
+
+/* Populations back to the rest equilibrium, u = 0, rho = 1, flags untouched
+ * (the state lbm_create leaves, lid included). Synchronous. */
+int lbm_reset(lbm_solver* solver);
+
 /* ---- misc ---------------------------------------------------------------- */
 
 int lbm_sync(lbm_solver* solver, hipStream_t stream);

@@ -11,7 +11,8 @@ def rt():
     global _L
     if _L is None:
         _L = ctypes.CDLL("libamdhip64.so.7")
-        for fn in ("hipMalloc", "hipFree", "hipMemcpy", "hipDeviceSynchronize"):
+        for fn in ("hipMalloc", "hipFree", "hipMemcpy", "hipDeviceSynchronize",
+                   "hipStreamCreate", "hipStreamDestroy", "hipStreamSynchronize"):
             getattr(_L, fn).restype = ctypes.c_int
     return _L
 
@@ -51,3 +52,15 @@ class DeviceBuffer:
             self.free()
         except Exception:
             pass
+
+
+def stream_create():
+    """A new (blocking, default-priority) HIP stream handle as an int."""
+    st = ctypes.c_void_p()
+    assert rt().hipStreamCreate(ctypes.byref(st)) == 0
+    return st.value
+
+
+def stream_destroy(st):
+    rt().hipStreamSynchronize(ctypes.c_void_p(st))
+    rt().hipStreamDestroy(ctypes.c_void_p(st))

@@ -1,0 +1,106 @@
+"""Batched cavities: every member is bit-identical to the same cavity stepped
+alone, for every launch shape, storage precision and with graph replay."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SPECS = [((17, 9, 12), 0.05, 100.0), ((1, 1, 1), 0.1, 10.0), ((2, 1, 3), 0.1, 20.0), ((40, 33, 21), 0.08, 300.0),
+         ((5, 64, 7), 0.03, 60.0), ((30, 30, 30), 0.1, 400.0)]
+
+
+def specs(hip, precision):
+    from cavityflow_cuda_amd.batch import CavitySpec
+    out = []
+    for size, u, re in SPECS:
+        out.append(CavitySpec(size, u, re, hip.cavity_parameters(size, u, re, precision)))
+    return out
+
+
+def alone(hip, sp, precision, steps):
+    from cavityflow_cuda_amd.batch import solve_alone
+    m, _ = solve_alone(sp, steps, precision)
+    return m
+
+
+TUNINGS = [dict(block=256, cells_per_thread=1, waves_per_cu=0, options=(1, 2)),
+           dict(block=512, cells_per_thread=2, waves_per_cu=4, options=(0, 0)),
+           dict(block=256, cells_per_thread=4, waves_per_cu=0, options=(3, 3)),
+           dict(block=512, cells_per_thread=1, waves_per_cu=16, options=(2, 1))]
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64", "f16s", "bf16s"])
+@pytest.mark.parametrize("tuning", range(len(TUNINGS)))
+def test_batch_members_equal_solo_runs(hip, precision, tuning):
+    from cavityflow_cuda_amd.batch import CavityBatch
+    print("[case] batch", precision, TUNINGS[tuning], flush=True)
+    steps = 37
+    sp = specs(hip, precision)
+    b = CavityBatch(sp, precision)
+    b.set_tuning(**TUNINGS[tuning])
+    b.run(steps, chunk=8)
+    b.sync()
+    for i, s in enumerate(sp):
+        want = alone(hip, s, precision, steps)
+        assert b.moments(i).tobytes() == want.tobytes(), (precision, i)
+    b.close()
+
+
+def test_graph_replay_equals_direct_launches(hip):
+    from hiprt import stream_create, stream_destroy
+    from cavityflow_cuda_amd.batch import CavityBatch
+    print("[case] batch graph replay", flush=True)
+    st = stream_create()
+    sp = specs(hip, "f32")
+    a = CavityBatch(sp, "f32")
+    b = CavityBatch(sp, "f32")
+    for _ in range(4):                    # replay the same graph several times,
+        a.run(10, stream=st, chunk=5)     # with host syncs and other work between
+        a.sync(st)
+        b.run(10, chunk=5)
+        b.sync()
+        for i in range(len(sp)):
+            assert a.moments(i).tobytes() == b.moments(i).tobytes()
+    a.close()
+    b.close()
+    stream_destroy(st)
+
+
+def test_autotune_then_reset_restores_initial_state(hip):
+    from cavityflow_cuda_amd.batch import CavityBatch
+    print("[case] batch autotune", flush=True)
+    sp = specs(hip, "f32")
+    fresh = CavityBatch(sp, "f32")
+    b = CavityBatch(sp, "f32")
+    t = b.autotune(trial_steps=2)
+    assert t["block"] in (256, 512) and t["cells_per_thread"] in (1, 2, 4)
+    for i in range(len(sp)):
+        assert b.moments(i).tobytes() == fresh.moments(i).tobytes()
+    b.run(20)
+    fresh.run(20)
+    b.sync()
+    fresh.sync()
+    for i in range(len(sp)):
+        assert b.moments(i).tobytes() == fresh.moments(i).tobytes()
+    b.close()
+    fresh.close()
+
+
+def test_batch_residual_matches_solver_residual(hip):
+    from cavityflow_cuda_amd.batch import CavityBatch
+    print("[case] batch residual", flush=True)
+    sp = specs(hip, "f32")
+    b = CavityBatch(sp, "f32")
+    c = CavityBatch(sp, "f32")
+    r0 = b.residuals()
+    assert np.isinf(r0).all()
+    for s in c.solvers:
+        assert np.isinf(s.residual(0))
+    b.run(50)
+    c.run(50)
+    r1 = b.residuals()
+    r2 = np.array([s.residual(0) for s in c.solvers])
+    finite = np.isfinite(r2)
+    assert np.allclose(r1[finite], r2[finite], rtol=1e-9, atol=0)
+    b.close()
+    c.close()
