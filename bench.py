@@ -24,6 +24,7 @@ from __future__ import annotations
 import argparse
 import ctypes
 import json
+import math
 import os
 import sys
 import time
@@ -224,6 +225,158 @@ def precision_workload(args, cf, L, hip, stream):
     print(json.dumps(line), flush=True)
 
 
+def batched_workload(args, cf, L, hip, stream):
+    """64 independent cavities (edges log-uniform in [32, 192], aspect <= 2,
+    Re uniform in [100, 2000], lid uniform in [0.02, 0.1], seeded) stepped
+    together by the batch launch with autotuned shapes and graph replay, in
+    FP32 and shifted FP16, against plain sequential solves of the same
+    cavities; every member is compared bit for bit with its solo run."""
+    from cavityflow_cuda_amd.batch import CavityBatch, draw_cavities, solve_alone
+    steps = args.steps
+    runs = {}
+    specs, redrawn = draw_cavities(args.cavities, args.seed)
+    cells = sum(sp.cells for sp in specs)
+    print(f"[batched] {len(specs)} cavities, {cells / 1e6:.2f} M interior cells, "
+          f"{redrawn} draws redrawn, {steps} steps", file=sys.stderr, flush=True)
+    for prec in ("f32", "f16s"):
+        for sp in specs:
+            sp.params = cf.cavity_parameters(sp.size, sp.u_lid, sp.reynolds, prec)
+        b = CavityBatch(specs, prec)
+        tuning = b.autotune(trial_steps=3, stream=stream.value)
+        print(f"[batched] {prec} tuning {tuning}", file=sys.stderr, flush=True)
+        b.run(100, stream=stream.value)      # warm-up: graph capture, then back to the start
+        hip.sync(stream)
+        b.reset()
+        e0, e1 = hip.event(), hip.event()
+        hip.record(e0, stream)
+        b.run(steps, stream=stream.value)
+        hip.record(e1, stream)
+        ms = hip.elapsed_ms(e0, e1)
+        batched = cells * steps / (ms * 1e-3) / 1e6
+        print(f"[batched] {prec} batch {ms:.1f} ms, {batched:.0f} MLUPS", file=sys.stderr,
+              flush=True)
+        seq_s, same = 0.0, 0
+        for i, sp in enumerate(specs):
+            m, dt = solve_alone(sp, steps, prec, stream=stream.value)
+            seq_s += dt
+            same += int(m.tobytes() == b.moments(i).tobytes())
+        seq = cells * steps / seq_s / 1e6
+        print(f"[batched] {prec} sequential {seq_s:.2f} s, {seq:.0f} MLUPS, "
+              f"{same}/{len(specs)} bitwise equal", file=sys.stderr, flush=True)
+        pop = BYTES_PER_POP[prec]
+        launch_ms = ms / steps                # one launch per time step
+        runs[prec] = dict(mlups=round(batched, 1), sequential_mlups=round(seq, 1),
+                          speedup=round(batched / seq, 3), bitwise_equal=same,
+                          batch_ms=round(ms, 2), sequential_s=round(seq_s, 3), tuning=tuning,
+                          mean_launch_ms=round(launch_ms, 5),
+                          hbm_gbs_alg=round(cells * 2 * 19 * pop / (launch_ms * 1e-3) / 1e9, 1))
+        b.close()
+    r = runs["f32"]
+    line = {
+        "metric": "aggregate MLUPS, batched cavities",
+        "value": r["mlups"], "unit": "MLUPS", "n_gpus": 1, "steps": steps, "warmup": 100,
+        "ms_per_step": round(r["batch_ms"] / steps, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{len(specs)} lid-driven cavities, edges log-uniform [32, 192], "
+                               f"aspect <= 2, Re U[100, 2000], lid U[0.02, 0.1], seed {args.seed}, "
+                               f"{steps} steps each", "interior_cells": cells,
+                   "redrawn": redrawn},
+        "roofline": {"bound": "hbm", "achieved": r["hbm_gbs_alg"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(r["hbm_gbs_alg"] / HBM_PEAK_GBS, 4),
+                     "traffic": None},
+        "runs": runs,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def serving_workload(args, cf, L, hip, stream):
+    """Continuous-batching server against plain sequential solves on the same
+    Poisson arrival trace (mean rate 80 % of the server's measured capacity):
+    edges log-uniform in [48, 384] (aspect <= 2), Re uniform in [100, 3200],
+    lid 0.1, each request to a residual of 1e-6 or the step cap, admission
+    under an HBM budget."""
+    from cavityflow_cuda_amd.batch import CavityBatch, CavitySpec
+    from cavityflow_cuda_amd import serving as sv
+    reqs, redrawn = sv.draw_requests(args.requests, args.seed, max_steps=args.max_steps)
+    work = np.array([r.cells * r.max_steps for r in reqs], dtype=np.float64)
+    print(f"[serving] {len(reqs)} requests, mean {np.mean([r.cells for r in reqs]) / 1e6:.2f} M "
+          f"cells, cap {args.max_steps} steps, {work.sum() / 1e12:.2f} T cell updates at the cap",
+          file=sys.stderr, flush=True)
+    # capacity: the batch rate on a sample of the requests, with autotuned shapes
+    sample = [CavitySpec(r.size, r.u_lid, r.reynolds, r.params) for r in reqs[:24]]
+    b = CavityBatch(sample, "f32")
+    tuning = b.autotune(trial_steps=3, stream=stream.value)
+    b.run(100, stream=stream.value)
+    hip.sync(stream)
+    e0, e1 = hip.event(), hip.event()
+    hip.record(e0, stream)
+    b.run(300, stream=stream.value)
+    hip.record(e1, stream)
+    cal = b.cells * 300 / (hip.elapsed_ms(e0, e1) * 1e-3)
+    b.close()
+    capacity = cal / work.mean()
+    rate = args.load * capacity
+    if args.load > 0:
+        sv.set_poisson_arrivals(reqs, rate, args.seed)
+    else:                       # closed loop: every request waiting at t = 0
+        rate = math.inf
+    print(f"[serving] calibration {cal / 1e6:.0f} MLUPS -> capacity {capacity:.3f} solves/s, "
+          f"arrival rate {rate:.3f}/s, tuning {tuning}", file=sys.stderr, flush=True)
+
+    def prog(srv):
+        print(f"[serving] t={srv.clock():.0f}s done {len(srv.results)} active {len(srv.active)} "
+              f"queued {len(srv.queue)} resident {srv.used / 1e9:.1f} GB", file=sys.stderr,
+              flush=True)
+
+    srv = sv.CavityServer("f32", hbm_budget=args.hbm_budget * 1e9, check_every=100,
+                          stream=stream.value, tuning=tuning)
+    served = srv.run(reqs, progress=prog)
+    peak_gb = srv.peak_used / 1e9
+    rounds = srv.rounds
+    srv.close()
+    s_sum = sv.latency_summary(served)
+    s_lu = sum(reqs[i].cells * r.steps for i, r in served.items() if r.status != "rejected")
+    s_mlups = s_lu / s_sum["span_s"] / 1e6
+    print(f"[serving] served {s_sum}", file=sys.stderr, flush=True)
+
+    def prog2(done, total):
+        print(f"[serving] sequential done {done}/{total}", file=sys.stderr, flush=True)
+
+    plain = sv.solve_sequential(reqs, "f32", 100, stream=stream.value, progress=prog2)
+    p_sum = sv.latency_summary(plain)
+    p_lu = sum(reqs[i].cells * r.steps for i, r in plain.items() if r.status != "rejected")
+    p_mlups = p_lu / p_sum["span_s"] / 1e6
+    print(f"[serving] plain {p_sum}", file=sys.stderr, flush=True)
+    same = sum(int(served[i].steps == plain[i].steps and
+                   served[i].totals.tobytes() == plain[i].totals.tobytes()) for i in served)
+    statuses = {}
+    for r in served.values():
+        statuses[r.status] = statuses.get(r.status, 0) + 1
+    line = {
+        "metric": "served cavity solves/s (p50/p99 latency) vs plain sequential",
+        "value": round(s_sum["solves_per_s"], 4), "unit": "solves/s", "n_gpus": 1,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{len(reqs)} cavity solve requests, " +
+                               (f"Poisson arrivals at {args.load:.0%} of measured capacity"
+                                if args.load > 0 else "all arriving at once") +
+                               ", edges log-uniform [48, 384] "
+                               f"(aspect <= 2), Re U[100, 3200], lid 0.1, residual 1e-6 or "
+                               f"{args.max_steps} steps, HBM budget {args.hbm_budget:.0f} GB, "
+                               f"seed {args.seed}", "redrawn": redrawn,
+                   "arrival_rate_per_s": round(rate, 4) if math.isfinite(rate) else None},
+        "served": dict(s_sum, aggregate_mlups=round(s_mlups, 1), peak_resident_gb=round(peak_gb, 2),
+                       rounds=rounds, statuses=statuses),
+        "plain_sequential": dict(p_sum, aggregate_mlups=round(p_mlups, 1)),
+        "throughput_ratio": round(s_sum["solves_per_s"] / p_sum["solves_per_s"], 3),
+        "mlups_ratio": round(s_mlups / p_mlups, 3),
+        "p99_ratio_plain_over_served": round(p_sum["p99_s"] / s_sum["p99_s"], 3),
+        "bitwise_equal_results": f"{same}/{len(served)}",
+        "calibration_mlups": round(cal / 1e6, 1),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -243,8 +396,17 @@ def main():
     ap.add_argument("--launch-alpha", type=int, default=None,
                     help="LBM_LAUNCH_* bits for the alpha kernel (default: library default)")
     ap.add_argument("--launch-beta", type=int, default=None)
-    ap.add_argument("--workload", default="step", choices=("step", "precision"))
+    ap.add_argument("--workload", default="step",
+                    choices=("step", "precision", "batched", "serving"))
     ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--cavities", type=int, default=64, help="batched: number of cavities")
+    ap.add_argument("--requests", type=int, default=200, help="serving: number of requests")
+    ap.add_argument("--max-steps", type=int, default=2000,
+                    help="serving: time-step cap per request")
+    ap.add_argument("--load", type=float, default=0.8,
+                    help="serving: arrival rate as a fraction of measured capacity "
+                         "(0: all requests arrive at once)")
+    ap.add_argument("--hbm-budget", type=float, default=200.0, help="serving: GB")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -263,6 +425,14 @@ def main():
     stream = hip.stream()
     if args.workload == "precision":
         precision_workload(args, cf, L, hip, stream)
+        return
+    if args.workload == "batched":
+        if args.steps == ap.get_default("steps"):
+            args.steps = 5000
+        batched_workload(args, cf, L, hip, stream)
+        return
+    if args.workload == "serving":
+        serving_workload(args, cf, L, hip, stream)
         return
     size = (args.size,) * 3
     params = cf.cavity_parameters(size, args.u_lid, args.reynolds, args.precision)

@@ -29,7 +29,7 @@ EXPORTED = (
     "lbm_set_launch_options", "lbm_get_launch_options", "lbm_set_step_pack", "lbm_get_moments",
     "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_size", "lbm_batch_set_tuning",
     "lbm_batch_get_tuning", "lbm_batch_step", "lbm_batch_run", "lbm_batch_autotune",
-    "lbm_batch_residual", "lbm_reset",
+    "lbm_batch_residual", "lbm_reset", "lbm_device_bytes",
     "lbm_reduce_box", "lbm_residual", "lbm_sync", "lbm_last_error", "lbm_status_string",
     "lbm_abi_version",
 )
@@ -160,6 +160,7 @@ def lib() -> ctypes.CDLL:
         "lbm_batch_autotune": ([P, I, P, ctypes.POINTER(lbm_batch_tuning)], I),
         "lbm_batch_residual": ([P, I, ctypes.POINTER(ctypes.c_double), P], I),
         "lbm_reset": ([P], I),
+        "lbm_device_bytes": ([ctypes.POINTER(lbm_params), ctypes.POINTER(ctypes.c_uint64)], I),
         "lbm_reduce_box": ([P, i3, i3, ctypes.POINTER(ctypes.c_double), P], I),
         "lbm_residual": ([P, I, ctypes.POINTER(ctypes.c_double), P], I),
         "lbm_sync": ([P, P], I),

@@ -124,15 +124,19 @@ class CavityBatch:
         self.steps = 0
 
     # -- stepping -------------------------------------------------------------
-    def run(self, steps: int, stream=None, chunk: int = 100):
-        """`steps` whole steps (alpha + beta), replayed from a captured graph of
-        `chunk` steps on a non-default stream."""
+    def run(self, steps: int, stream=None, chunk: int = 200):
+        """`steps` time steps (alternating alpha and beta, as
+        CController::computeNextStep counts them; even, so every member ends
+        in the natural layout), replayed from a captured graph of `chunk`
+        steps on a non-default stream."""
+        if steps % 2 or chunk % 2 or chunk <= 0:
+            raise ValueError("steps and chunk must be even (whole alpha + beta pairs)")
         st = _stream(stream)
         full, rest = divmod(int(steps), chunk)
         for _ in range(full):
-            check(lib().lbm_batch_run(self._h, 0, 2 * chunk, st))
+            check(lib().lbm_batch_run(self._h, 0, chunk, st))
         if rest:
-            check(lib().lbm_batch_run(self._h, 0, 2 * rest, st))
+            check(lib().lbm_batch_run(self._h, 0, rest, st))
         self.steps += int(steps)
 
     def sync(self, stream=None):
@@ -149,8 +153,11 @@ class CavityBatch:
 
 
 def solve_alone(spec, steps, precision="f32", device=0, stream=None):
-    """One cavity on its own solver: per-half-step launches, no batching, no
-    graph (the plain sequential baseline). Returns (moments, seconds)."""
+    """One cavity on its own solver for `steps` (even) time steps: one launch
+    per step, no batching, no graph (the plain sequential baseline).
+    Returns (moments, seconds)."""
+    if steps % 2:
+        raise ValueError("steps must be even (whole alpha + beta pairs)")
     p = spec.params or cavity_parameters(spec.size, spec.u_lid, spec.reynolds, precision)
     s = LbmSolverHIP(spec.size, (_lib.OBSTACLE,) * 6, precision, p["inv_tau"],
                      p["velocity_dimless"][0], p["acceleration_dimless"], False, False, device)
@@ -159,7 +166,7 @@ def solve_alone(spec, steps, precision="f32", device=0, stream=None):
     st = _stream(stream)
     s.sync(stream)
     t0 = time.perf_counter()
-    for _ in range(int(steps)):
+    for _ in range(int(steps) // 2):
         check(L.lbm_simulation_step_alpha(s.handle, None, None, st))
         check(L.lbm_simulation_step_beta(s.handle, None, None, st))
     s.sync(stream)

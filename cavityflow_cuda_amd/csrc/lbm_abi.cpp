@@ -274,6 +274,60 @@ int lbm_compute_parameters(const lbm_physics* in, int precision, lbm_derived* ou
     return compute_parameters_t<float>(in, out);
 }
 
+namespace {
+
+// Shared validation of lbm_create / lbm_device_bytes: cells with halo.
+int check_params(const lbm_params& p, const char* who, int64_t* cells)
+{
+    if (!element_bytes(p.precision)) {
+        set_error(std::string(who) + ": unknown precision");
+        return LBM_ERR_ARG;
+    }
+    for (int i = 0; i < 3; i++)
+        if (p.size[i] < 1) {
+            set_error(std::string(who) + ": size must be >= 1 on every axis");
+            return LBM_ERR_ARG;
+        }
+    for (int i = 0; i < 6; i++)
+        if (!valid_flag(p.bc[i])) {
+            set_error(std::string(who) + ": boundary condition is not a Flag value");
+            return LBM_ERR_ARG;
+        }
+    const int64_t nx = p.size[0] + 2, ny = p.size[1] + 2, nz = p.size[2] + 2;
+    const int64_t n = nx * ny * nz;
+    // 32-bit cell indices with room for the largest neighbour offset
+    if (n + nx * ny + nx + 1 >= (int64_t)INT32_MAX) {
+        set_error(std::string(who) + ": domain too large for 32-bit cell indexing");
+        return LBM_ERR_ARG;
+    }
+    *cells = n;
+    return LBM_OK;
+}
+
+}  // namespace
+
+int lbm_device_bytes(const lbm_params* params, uint64_t* bytes)
+{
+    if (!params || !bytes) {
+        set_error("lbm_device_bytes: NULL argument");
+        return LBM_ERR_ARG;
+    }
+    int64_t n = 0;
+    int rc = check_params(*params, "lbm_device_bytes", &n);
+    if (rc)
+        return rc;
+    const uint64_t stride = (uint64_t)(n + 255) / 256 * 256;
+    const uint64_t eb = (uint64_t)element_bytes(params->precision);
+    const uint64_t ob = params->precision == LBM_F64 ? 8 : 4;
+    uint64_t b = (uint64_t)kQ * stride * eb + stride;                  // populations, flags
+    b += params->store_densities ? stride * ob : 0;
+    b += params->store_velocities ? 3 * stride * ob : 0;
+    b += (uint64_t)(kReduceBlocks + 1) * 5 * sizeof(double);          // reduction partials
+    b += 3 * stride * ob;                                              // residual snapshot
+    *bytes = b;
+    return LBM_OK;
+}
+
 int lbm_create(const lbm_params* params, lbm_solver** out)
 {
     if (!params || !out) {
@@ -282,28 +336,12 @@ int lbm_create(const lbm_params* params, lbm_solver** out)
     }
     *out = nullptr;
     const lbm_params& p = *params;
+    int64_t n = 0;
+    int rc0 = check_params(p, "lbm_create", &n);
+    if (rc0)
+        return rc0;
     const int eb = element_bytes(p.precision);
-    if (!eb) {
-        set_error("lbm_create: unknown precision");
-        return LBM_ERR_ARG;
-    }
-    for (int i = 0; i < 3; i++)
-        if (p.size[i] < 1) {
-            set_error("lbm_create: size must be >= 1 on every axis");
-            return LBM_ERR_ARG;
-        }
-    for (int i = 0; i < 6; i++)
-        if (!valid_flag(p.bc[i])) {
-            set_error("lbm_create: boundary condition is not a Flag value");
-            return LBM_ERR_ARG;
-        }
     const int64_t nx = p.size[0] + 2, ny = p.size[1] + 2, nz = p.size[2] + 2;
-    const int64_t n = nx * ny * nz;
-    // 32-bit cell indices with room for the largest neighbour offset
-    if (n + nx * ny + nx + 1 >= (int64_t)INT32_MAX) {
-        set_error("lbm_create: domain too large for 32-bit cell indexing");
-        return LBM_ERR_ARG;
-    }
     lbm_solver* s = new lbm_solver();
     std::memset(s, 0, sizeof(*s));
     s->p = p;
@@ -813,8 +851,7 @@ int lbm_residual(lbm_solver* s, int layout, double* residual, hipStream_t stream
         *residual = std::numeric_limits<double>::infinity();
         s->residual_valid = 1;
     } else {
-        *residual = host[1] > 0 ? std::sqrt(host[0] / host[1])
-                                : (host[0] > 0 ? std::numeric_limits<double>::infinity() : 0.0);
+        *residual = residual_ratio(host[0], host[1]);
     }
     return LBM_OK;
 }

@@ -624,8 +624,7 @@ int lbm_batch_residual(lbm_batch* b, int layout, double* residuals, hipStream_t 
             s->residual_valid = 1;
         } else {
             const double num = host[2 * i], den = host[2 * i + 1];
-            residuals[i] = den > 0 ? std::sqrt(num / den)
-                                   : (num > 0 ? std::numeric_limits<double>::infinity() : 0.0);
+            residuals[i] = residual_ratio(num, den);
         }
     }
     return LBM_OK;

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
+#include <limits>
 #include <string>
 
 #include "lbm_abi.h"
@@ -113,5 +115,16 @@ hipError_t launch_residual(const lbm_solver* s, int layout, double* partials, do
                            hipStream_t st);
 
 constexpr int kReduceBlocks = 1024;
+
+// ||u - u_prev|| / ||u|| from the two FP64 sums; a non-finite state gives NaN
+// (never a small number that would pass a convergence test).
+inline double residual_ratio(double num, double den)
+{
+    if (std::isnan(num) || std::isnan(den) || std::isinf(num) || std::isinf(den))
+        return std::numeric_limits<double>::quiet_NaN();
+    if (den > 0)
+        return std::sqrt(num / den);
+    return num > 0 ? std::numeric_limits<double>::infinity() : 0.0;
+}
 
 }  // namespace lbm

@@ -80,6 +80,32 @@ class SolverConfig:
     extra: dict = field(default_factory=dict)
 
 
+def make_params(size, bc=(_lib.OBSTACLE,) * 6, precision="f32", inv_tau=1.0, lid_velocity=0.0,
+                gravitation=(0.0, 0.0, 0.0), store_densities=True, store_velocities=True,
+                device=0):
+    """The lbm_params struct of one subdomain solver."""
+    p = _lib.lbm_params()
+    p.size[:] = [int(v) for v in size]
+    p.bc[:] = [int(v) for v in bc]
+    p.precision = precision_code(precision)
+    p.inv_tau = float(inv_tau)
+    p.lid_velocity = float(lid_velocity)
+    p.gravitation[:] = [float(v) for v in gravitation]
+    p.store_densities = int(bool(store_densities))
+    p.store_velocities = int(bool(store_velocities))
+    p.device = int(device)
+    return p
+
+
+def device_bytes(size, precision="f32", store_outputs=False) -> int:
+    """Device bytes of one solver of this shape, residual snapshot included."""
+    p = make_params(size, precision=precision, store_densities=store_outputs,
+                    store_velocities=store_outputs)
+    b = ctypes.c_uint64()
+    check(lib().lbm_device_bytes(ctypes.byref(p), ctypes.byref(b)))
+    return int(b.value)
+
+
 class LbmSolverHIP:
     """One subdomain on one GPU (CLbmSolverGPU<T> replacement)."""
 
@@ -87,16 +113,8 @@ class LbmSolverHIP:
                  lid_velocity=0.0, gravitation=(0.0, 0.0, 0.0), store_densities=True,
                  store_velocities=True, device=0):
         L = lib()
-        p = _lib.lbm_params()
-        p.size[:] = [int(v) for v in size]
-        p.bc[:] = [int(v) for v in bc]
-        p.precision = precision_code(precision)
-        p.inv_tau = float(inv_tau)
-        p.lid_velocity = float(lid_velocity)
-        p.gravitation[:] = [float(v) for v in gravitation]
-        p.store_densities = int(bool(store_densities))
-        p.store_velocities = int(bool(store_velocities))
-        p.device = int(device)
+        p = make_params(size, bc, precision, inv_tau, lid_velocity, gravitation,
+                        store_densities, store_velocities, device)
         self._h = ctypes.c_void_p()
         check(L.lbm_create(ctypes.byref(p), ctypes.byref(self._h)))
         self.params = p

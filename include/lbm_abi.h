@@ -288,7 +288,7 @@ int lbm_reduce_box(lbm_solver* solver, const int origin[3], const int size[3], d
  * where u is the momentum computed from the populations (`layout` 0: after
  * beta, natural slots; 1: after alpha, swapped slots) and u_prev the value
  * saved by the previous call on this solver. Saves u for the next call.
- * The first call returns +inf. Synchronous. */
+ * The first call returns +inf; a non-finite state returns NaN. Synchronous. */
 int lbm_residual(lbm_solver* solver, int layout, double* residual, hipStream_t stream);
 
 /* ---- batched cavities ----------------------------------------------------
@@ -329,6 +329,12 @@ int lbm_batch_residual(lbm_batch* batch, int layout, double* residuals, hipStrea
 /* Populations back to the rest equilibrium, u = 0, rho = 1, flags untouched
  * (the state lbm_create leaves, lid included). Synchronous. */
 int lbm_reset(lbm_solver* solver);
+
+/* Device bytes lbm_create allocates for these parameters, plus the residual
+ * snapshot lbm_residual / lbm_batch_residual add on first use (no
+ * counterpart in the reference; used for HBM-budget admission). Validates
+ * params like lbm_create; touches no device. */
+int lbm_device_bytes(const lbm_params* params, uint64_t* bytes);
 
 /* ---- misc ---------------------------------------------------------------- */
 

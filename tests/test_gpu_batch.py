@@ -34,7 +34,7 @@ TUNINGS = [dict(block=256, cells_per_thread=1, waves_per_cu=0, options=(1, 2)),
 def test_batch_members_equal_solo_runs(hip, precision, tuning):
     from cavityflow_cuda_amd.batch import CavityBatch
     print("[case] batch", precision, TUNINGS[tuning], flush=True)
-    steps = 37
+    steps = 38
     sp = specs(hip, precision)
     b = CavityBatch(sp, precision)
     b.set_tuning(**TUNINGS[tuning])
@@ -55,9 +55,9 @@ def test_graph_replay_equals_direct_launches(hip):
     a = CavityBatch(sp, "f32")
     b = CavityBatch(sp, "f32")
     for _ in range(4):                    # replay the same graph several times,
-        a.run(10, stream=st, chunk=5)     # with host syncs and other work between
+        a.run(20, stream=st, chunk=10)    # with host syncs and other work between
         a.sync(st)
-        b.run(10, chunk=5)
+        b.run(20, chunk=10)
         b.sync()
         for i in range(len(sp)):
             assert a.moments(i).tobytes() == b.moments(i).tobytes()
@@ -76,8 +76,8 @@ def test_autotune_then_reset_restores_initial_state(hip):
     assert t["block"] in (256, 512) and t["cells_per_thread"] in (1, 2, 4)
     for i in range(len(sp)):
         assert b.moments(i).tobytes() == fresh.moments(i).tobytes()
-    b.run(20)
-    fresh.run(20)
+    b.run(40)
+    fresh.run(40)
     b.sync()
     fresh.sync()
     for i in range(len(sp)):
@@ -96,8 +96,8 @@ def test_batch_residual_matches_solver_residual(hip):
     assert np.isinf(r0).all()
     for s in c.solvers:
         assert np.isinf(s.residual(0))
-    b.run(50)
-    c.run(50)
+    b.run(100)
+    c.run(100)
     r1 = b.residuals()
     r2 = np.array([s.residual(0) for s in c.solvers])
     finite = np.isfinite(r2)
