@@ -329,7 +329,7 @@ def serving_workload(args, cf, L, hip, stream):
               flush=True)
 
     srv = sv.CavityServer("f32", hbm_budget=args.hbm_budget * 1e9, check_every=100,
-                          stream=stream.value, tuning=tuning)
+                          stream=stream.value, tuning=tuning, policy=args.policy)
     served = srv.run(reqs, progress=prog)
     peak_gb = srv.peak_used / 1e9
     rounds = srv.rounds
@@ -363,7 +363,7 @@ def serving_workload(args, cf, L, hip, stream):
                                ", edges log-uniform [48, 384] "
                                f"(aspect <= 2), Re U[100, 3200], lid 0.1, residual 1e-6 or "
                                f"{args.max_steps} steps, HBM budget {args.hbm_budget:.0f} GB, "
-                               f"seed {args.seed}", "redrawn": redrawn,
+                               f"seed {args.seed}, policy {args.policy}", "redrawn": redrawn,
                    "arrival_rate_per_s": round(rate, 4) if math.isfinite(rate) else None},
         "served": dict(s_sum, aggregate_mlups=round(s_mlups, 1), peak_resident_gb=round(peak_gb, 2),
                        rounds=rounds, statuses=statuses),
@@ -407,6 +407,8 @@ def main():
                     help="serving: arrival rate as a fraction of measured capacity "
                          "(0: all requests arrive at once)")
     ap.add_argument("--hbm-budget", type=float, default=200.0, help="serving: GB")
+    ap.add_argument("--policy", default="srpt", choices=("srpt", "all"),
+                    help="serving: which resident solves each round advances")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

@@ -17,6 +17,8 @@ LEFT, RIGHT, BOTTOM, TOP, BACK, FRONT = range(6)
 F32, F64, F16S, BF16S = 0, 1, 2, 3
 LAUNCH_NONTEMPORAL, LAUNCH_XCD_REMAP = 1, 2
 LBM_OK = 0
+(LBM_ERR_ARG, LBM_ERR_BOX, LBM_ERR_HIP, LBM_ERR_NOMEM, LBM_ERR_PARAMS, LBM_ERR_STATE,
+ LBM_ERR_UNSUPPORTED) = range(-1, -8, -1)
 
 EXPORTED = (
     "lbm_compute_parameters", "lbm_create", "lbm_destroy", "lbm_get_device_arrays",
@@ -87,6 +89,7 @@ class lbm_batch_tuning(ctypes.Structure):
         ("cells_per_thread", ctypes.c_int),
         ("waves_per_cu", ctypes.c_int),
         ("options", ctypes.c_int * 2),
+        ("pairs", ctypes.c_int),
     ]
 
 
@@ -160,7 +163,7 @@ def lib() -> ctypes.CDLL:
         "lbm_batch_autotune": ([P, I, P, ctypes.POINTER(lbm_batch_tuning)], I),
         "lbm_batch_residual": ([P, I, ctypes.POINTER(ctypes.c_double), P], I),
         "lbm_reset": ([P], I),
-        "lbm_device_bytes": ([ctypes.POINTER(lbm_params), ctypes.POINTER(ctypes.c_uint64)], I),
+        "lbm_device_bytes": ([ctypes.POINTER(lbm_params), I, ctypes.POINTER(ctypes.c_uint64)], I),
         "lbm_reduce_box": ([P, i3, i3, ctypes.POINTER(ctypes.c_double), P], I),
         "lbm_residual": ([P, I, ctypes.POINTER(ctypes.c_double), P], I),
         "lbm_sync": ([P, P], I),

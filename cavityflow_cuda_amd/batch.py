@@ -104,10 +104,14 @@ class CavityBatch:
         t = _lib.lbm_batch_tuning()
         check(lib().lbm_batch_get_tuning(self._h, ctypes.byref(t)))
         return dict(block=t.block, cells_per_thread=t.cells_per_thread,
-                    waves_per_cu=t.waves_per_cu, options=(t.options[0], t.options[1]))
+                    waves_per_cu=t.waves_per_cu, options=(t.options[0], t.options[1]),
+                    pairs=t.pairs)
 
-    def set_tuning(self, block=256, cells_per_thread=1, waves_per_cu=0, options=(1, 2)):
-        t = _lib.lbm_batch_tuning(block, cells_per_thread, waves_per_cu, (ctypes.c_int * 2)(*options))
+    def set_tuning(self, block=256, cells_per_thread=1, waves_per_cu=0, options=(1, 2), pairs=0):
+        """Launch shape; pairs=1 runs each alpha + beta pair as one out-of-place
+        pass (then run() needs multiples of 4 steps)."""
+        t = _lib.lbm_batch_tuning(block, cells_per_thread, waves_per_cu,
+                                  (ctypes.c_int * 2)(*options), int(pairs))
         check(lib().lbm_batch_set_tuning(self._h, ctypes.byref(t)))
 
     def autotune(self, trial_steps=3, stream=None) -> dict:

@@ -306,7 +306,7 @@ int check_params(const lbm_params& p, const char* who, int64_t* cells)
 
 }  // namespace
 
-int lbm_device_bytes(const lbm_params* params, uint64_t* bytes)
+int lbm_device_bytes(const lbm_params* params, int pair_buffer, uint64_t* bytes)
 {
     if (!params || !bytes) {
         set_error("lbm_device_bytes: NULL argument");
@@ -324,6 +324,7 @@ int lbm_device_bytes(const lbm_params* params, uint64_t* bytes)
     b += params->store_velocities ? 3 * stride * ob : 0;
     b += (uint64_t)(kReduceBlocks + 1) * 5 * sizeof(double);          // reduction partials
     b += 3 * stride * ob;                                              // residual snapshot
+    b += pair_buffer ? (uint64_t)kQ * stride * eb : 0;                 // second populations
     *bytes = b;
     return LBM_OK;
 }
@@ -404,7 +405,8 @@ int lbm_destroy(lbm_solver* s)
     if (cur != s->p.device)
         (void)hipSetDevice(s->p.device);
     (void)hipDeviceSynchronize();
-    void* bufs[] = {s->dd, s->flags, s->rho, s->u, s->staging, s->partials, s->residual_prev};
+    void* bufs[] = {s->dd,       s->flags,    s->rho,           s->u,
+                    s->staging,  s->partials, s->residual_prev, s->dd_alt};
     for (void* b : bufs)
         if (b)
             (void)hipFree(b);

@@ -7,8 +7,10 @@
 // Work is cut into tiles of block * cells_per_thread consecutive cells of one
 // member; tile_member[t] names the member of tile t. Either one workgroup per
 // tile (optionally renumbered per XCD) or a persistent grid of
-// waves_per_cu * CUs workgroups striding over the tiles. An autotuner times
-// the candidate shapes on the actual batch and keeps the fastest.
+// waves_per_cu * CUs workgroups striding over the tiles. With the pair pass
+// (lbm_pair.hpp) one launch runs alpha and beta of every member out of place
+// between each member's two population buffers. An autotuner times the
+// candidate shapes on the actual batch and keeps the fastest.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -19,6 +21,7 @@
 
 #include "lbm_abi.h"
 #include "lbm_cell.hpp"
+#include "lbm_pair.hpp"
 
 namespace lbm {
 
@@ -96,7 +99,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch(const MemberDesc* __restrict__ 
                 const uint32_t id = base + (uint32_t)k * BLOCK + threadIdx.x;
                 if (id < (uint32_t)a.n) {
                     const Acc32<S, NT> acc{a.plane, id * (uint32_t)sizeof(S)};
-                    alpha_cell<P>(a, id, acc);
+                    alpha_cell<P>(a, id, a.flags[id], acc);
                 }
             }
         } else {
@@ -110,10 +113,10 @@ __global__ __launch_bounds__(BLOCK) void k_batch(const MemberDesc* __restrict__ 
                 if (wrap) {
                     // the wrap path addresses planes from the unshifted base
                     const Acc64<S, true> acc{(S*)d.plane[0][0], a.stride, (int32_t)id, a.n};
-                    beta_cell<P>(a, id, acc);
+                    beta_cell<P>(a, id, a.flags[id], acc);
                 } else {
                     const Acc32<S, false> acc{a.plane, id * (uint32_t)sizeof(S)};
-                    beta_cell<P>(a, id, acc);
+                    beta_cell<P>(a, id, a.flags[id], acc);
                 }
             }
         }
@@ -211,6 +214,10 @@ struct lbm_batch {
     int32_t* res_member = nullptr;
     MemberDesc* res_desc = nullptr;
     double* res_out = nullptr;
+    PairDesc* pair_desc = nullptr;     // pair pass tables (tuning.pairs)
+    int32_t* pair_member = nullptr;
+    uint32_t pair_tiles = 0;
+    int* check_flag = nullptr;         // boundary check result
     std::vector<GraphEntry> graphs;
 };
 
@@ -260,19 +267,149 @@ int upload_tiles(lbm_batch* b, uint32_t tile_cells, std::vector<MemberDesc>& des
 
 bool valid_tuning(const lbm_batch_tuning& t)
 {
-    return (t.block == 256 || t.block == 512) &&
+    return t.pairs >= 0 && t.pairs <= kPairShapes && (t.block == 256 || t.block == 512) &&
            (t.cells_per_thread == 1 || t.cells_per_thread == 2 || t.cells_per_thread == 4) &&
            t.waves_per_cu >= 0 && t.waves_per_cu <= 64 &&
            !(t.options[0] & ~(LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_XCD_REMAP)) &&
            !(t.options[1] & ~(LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_XCD_REMAP));
 }
 
+// Pair pass set-up: precision and boundary checks, second population
+// buffers, and the column-tile tables.
+void pair_tile(int pairs, int* tx, int* ty)
+{
+    switch (pairs) {
+    case 2: *tx = PairShape1::kTX; *ty = PairShape1::kTY; break;
+    case 3: *tx = PairShape2::kTX; *ty = PairShape2::kTY; break;
+    case 4: *tx = PairShape3::kTX; *ty = PairShape3::kTY; break;
+    default: *tx = PairShape0::kTX; *ty = PairShape0::kTY; break;
+    }
+}
+
+int prepare_pairs(lbm_batch* b, int pairs)
+{
+    int ptx, pty;
+    pair_tile(pairs, &ptx, &pty);
+    if (b->precision == LBM_F64) {
+        set_error("batch pair pass: FP64 is not supported (LDS window too large)");
+        return LBM_ERR_UNSUPPORTED;
+    }
+    for (const lbm_solver* s : b->members)
+        if (s->p.store_densities || s->p.store_velocities) {
+            set_error("batch pair pass: members must not store rho / u outputs");
+            return LBM_ERR_UNSUPPORTED;
+        }
+    if (!b->check_flag && hipMalloc((void**)&b->check_flag, sizeof(int)) != hipSuccess) {
+        set_error("batch pair pass: cannot allocate the check flag");
+        return LBM_ERR_NOMEM;
+    }
+    std::vector<PairDesc> desc(b->members.size());
+    std::vector<int32_t> map;
+    for (size_t m = 0; m < b->members.size(); m++) {
+        lbm_solver* s = b->members[m];
+        const int nx = s->size_h[0], ny = s->size_h[1], nz = s->size_h[2];
+        BATCH_HIP(hipMemset(b->check_flag, 0, sizeof(int)), "hipMemset");
+        hipLaunchKernelGGL(k_boundary_check, dim3(std::min<int64_t>((s->n + 255) / 256, 4096)),
+                           dim3(256), 0, nullptr, s->flags, nx, ny, nz, b->check_flag);
+        BATCH_HIP(hipGetLastError(), "boundary check launch");
+        int bad = 0;
+        BATCH_HIP(hipMemcpy(&bad, b->check_flag, sizeof(int), hipMemcpyDeviceToHost),
+                  "D2H copy");
+        if (bad) {
+            set_error("batch pair pass: a member has a non-OBSTACLE cell on its array boundary");
+            return LBM_ERR_UNSUPPORTED;
+        }
+        if (!s->dd_alt &&
+            hipMalloc(&s->dd_alt, (size_t)kQ * s->stride * s->elem_bytes) != hipSuccess) {
+            set_error("batch pair pass: cannot allocate the second population buffer");
+            return LBM_ERR_NOMEM;
+        }
+        PairDesc& d = desc[m];
+        const MemberDesc& h = b->host_desc[m];
+        d.buf[0] = (const char*)s->dd;
+        d.buf[1] = (const char*)s->dd_alt;
+        d.flags = s->flags;
+        d.stride = s->stride;
+        d.nx = nx;
+        d.ny = ny;
+        d.nz = nz;
+        d.tiles_x = (nx + ptx - 1) / ptx;
+        d.tiles_y = (ny + pty - 1) / pty;
+        d.chunks = (nz + kPairZChunk - 1) / kPairZChunk;
+        d.tile_begin = (uint32_t)map.size();
+        d.inv_tau = h.inv_tau;
+        d.lid = h.lid;
+        d.g0 = h.g0;
+        d.g1 = h.g1;
+        d.g2 = h.g2;
+        map.insert(map.end(), (size_t)d.tiles_x * d.tiles_y * d.chunks, (int32_t)m);
+    }
+    if (b->pair_member)
+        BATCH_HIP(hipFree(b->pair_member), "hipFree");
+    if (b->pair_desc)
+        BATCH_HIP(hipFree(b->pair_desc), "hipFree");
+    b->pair_member = nullptr;
+    b->pair_desc = nullptr;
+    if (hipMalloc((void**)&b->pair_member, map.size() * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc((void**)&b->pair_desc, desc.size() * sizeof(PairDesc)) != hipSuccess) {
+        set_error("batch pair pass: cannot allocate the tile table");
+        return LBM_ERR_NOMEM;
+    }
+    BATCH_HIP(hipMemcpy(b->pair_member, map.data(), map.size() * sizeof(int32_t),
+                        hipMemcpyHostToDevice), "H2D tile table");
+    BATCH_HIP(hipMemcpy(b->pair_desc, desc.data(), desc.size() * sizeof(PairDesc),
+                        hipMemcpyHostToDevice), "H2D member table");
+    b->pair_tiles = (uint32_t)map.size();
+    return LBM_OK;
+}
+
 int apply_tuning(lbm_batch* b, const lbm_batch_tuning& t)
 {
     drop_graphs(b);
+    if (t.pairs) {
+        int rc = prepare_pairs(b, t.pairs);
+        if (rc)
+            return rc;
+    }
     b->tuning = t;
     return upload_tiles(b, (uint32_t)(t.block * t.cells_per_thread), b->host_desc,
                         &b->tile_member, &b->tiles, &b->desc);
+}
+
+template <class P, class G>
+hipError_t launch_pair_g(const lbm_batch* b, int parity, hipStream_t st)
+{
+    const int xcd = (b->tuning.options[0] & LBM_LAUNCH_XCD_REMAP) ? 1 : 0;
+    const dim3 grid(b->pair_tiles), block(G::kBlock);
+    if (b->tuning.options[0] & LBM_LAUNCH_NONTEMPORAL)
+        hipLaunchKernelGGL((k_pair<P, true, G>), grid, block, 0, st, b->pair_desc,
+                           b->pair_member, b->pair_tiles, parity, xcd);
+    else
+        hipLaunchKernelGGL((k_pair<P, false, G>), grid, block, 0, st, b->pair_desc,
+                           b->pair_member, b->pair_tiles, parity, xcd);
+    return hipGetLastError();
+}
+
+template <class P>
+hipError_t launch_pair_p(const lbm_batch* b, int parity, hipStream_t st)
+{
+    switch (b->tuning.pairs) {
+    case 1: return launch_pair_g<P, PairShape0>(b, parity, st);
+    case 2: return launch_pair_g<P, PairShape1>(b, parity, st);
+    case 3: return launch_pair_g<P, PairShape2>(b, parity, st);
+    case 4: return launch_pair_g<P, PairShape3>(b, parity, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_pair(const lbm_batch* b, int parity, hipStream_t st)
+{
+    switch (b->precision) {
+    case LBM_F32: return launch_pair_p<StoreF32>(b, parity, st);
+    case LBM_F16S: return launch_pair_p<StoreF16S>(b, parity, st);
+    case LBM_BF16S: return launch_pair_p<StoreBF16S>(b, parity, st);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 template <class P, int BLOCK>
@@ -310,6 +447,21 @@ hipError_t launch_phase(const lbm_batch* b, int phase, hipStream_t st)
     case LBM_BF16S: return launch_phase_p<StoreBF16S>(b, phase, st);
     default: return hipErrorInvalidValue;
     }
+}
+
+// The launches of half_steps half-steps from first_phase: single half-steps,
+// or one pair pass per alpha + beta pair alternating between the buffers.
+hipError_t enqueue(const lbm_batch* b, int first_phase, int half_steps, hipStream_t st)
+{
+    hipError_t e = hipSuccess;
+    if (b->tuning.pairs) {
+        for (int i = 0; i < half_steps / 2 && e == hipSuccess; i++)
+            e = launch_pair(b, i & 1, st);
+        return e;
+    }
+    for (int i = 0; i < half_steps && e == hipSuccess; i++)
+        e = launch_phase(b, (first_phase + i) & 1, st);
+    return e;
 }
 
 int set_dev(int device)
@@ -391,6 +543,7 @@ int lbm_batch_create(lbm_solver* const* members, int count, lbm_batch** out)
     t.waves_per_cu = 0;
     t.options[0] = LBM_LAUNCH_NONTEMPORAL;
     t.options[1] = LBM_LAUNCH_XCD_REMAP;
+    t.pairs = 0;
     if ((rc = apply_tuning(b, t))) {
         lbm_batch_destroy(b);
         return rc;
@@ -406,7 +559,8 @@ int lbm_batch_destroy(lbm_batch* b)
     (void)set_dev(b->device);
     (void)hipDeviceSynchronize();
     drop_graphs(b);
-    void* bufs[] = {b->desc, b->tile_member, b->res_member, b->res_desc, b->res_out};
+    void* bufs[] = {b->desc,    b->tile_member, b->res_member, b->res_desc,
+                    b->res_out, b->pair_desc,   b->pair_member, b->check_flag};
     for (void* p : bufs)
         if (p)
             (void)hipFree(p);
@@ -460,11 +614,15 @@ int lbm_batch_run(lbm_batch* b, int first_phase, int half_steps, hipStream_t str
     int rc = set_dev(b->device);
     if (rc)
         return rc;
+    if (b->tuning.pairs && (first_phase != 0 || half_steps % 4)) {
+        set_error("lbm_batch_run: the pair pass needs first_phase 0 and a multiple of 4 "
+                  "half-steps");
+        return LBM_ERR_ARG;
+    }
     if (half_steps == 0)
         return LBM_OK;
     if (!stream) {   // the legacy default stream cannot be captured
-        for (int i = 0; i < half_steps; i++)
-            BATCH_HIP(launch_phase(b, (first_phase + i) & 1, nullptr), "batch kernel launch");
+        BATCH_HIP(enqueue(b, first_phase, half_steps, nullptr), "batch kernel launch");
         return LBM_OK;
     }
     for (auto& g : b->graphs)
@@ -474,9 +632,7 @@ int lbm_batch_run(lbm_batch* b, int first_phase, int half_steps, hipStream_t str
         }
     hipGraph_t graph;
     BATCH_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
-    hipError_t le = hipSuccess;
-    for (int i = 0; i < half_steps && le == hipSuccess; i++)
-        le = launch_phase(b, (first_phase + i) & 1, stream);
+    const hipError_t le = enqueue(b, first_phase, half_steps, stream);
     hipError_t ee = hipStreamEndCapture(stream, &graph);
     if (le != hipSuccess)
         return hip_fail(le, "batch kernel capture");
@@ -502,17 +658,29 @@ int lbm_batch_autotune(lbm_batch* b, int trial_steps, hipStream_t stream,
     int rc = set_dev(b->device);
     if (rc)
         return rc;
-    // Each phase is tuned on its own; the members' state advances by whole
-    // alpha+beta steps during the trials, exactly as in a run.
+    // Each phase of the in-place launches is tuned on its own; the pair pass
+    // is timed as a whole. The members' state advances by whole alpha + beta
+    // steps during the trials (an even number of pair passes), exactly as in
+    // a run. Times are per whole step, the first trial step left out.
     hipEvent_t e0, e1;
     BATCH_HIP(hipEventCreate(&e0), "hipEventCreate");
     BATCH_HIP(hipEventCreate(&e1), "hipEventCreate");
+    auto timed = [&](auto&& launch, float* ms) -> int {
+        BATCH_HIP(hipEventRecord(e0, stream), "hipEventRecord");
+        BATCH_HIP(launch(), "batch kernel launch");
+        BATCH_HIP(hipEventRecord(e1, stream), "hipEventRecord");
+        BATCH_HIP(hipEventSynchronize(e1), "hipEventSynchronize");
+        BATCH_HIP(hipEventElapsedTime(ms, e0, e1), "hipEventElapsedTime");
+        return LBM_OK;
+    };
     const int blocks[] = {256, 512};
     const int cpts[] = {1, 2, 4};
     const int wpcs[] = {0, 4, 8, 16};
     lbm_batch_tuning best = b->tuning;
+    best.pairs = 0;
     float best_ms[2] = {std::numeric_limits<float>::infinity(), std::numeric_limits<float>::infinity()};
     lbm_batch_tuning cand;
+    cand.pairs = 0;
     for (int blk : blocks)
         for (int cpt : cpts)
             for (int wpc : wpcs)
@@ -526,12 +694,9 @@ int lbm_batch_autotune(lbm_batch* b, int trial_steps, hipStream_t stream,
                     float ms[2] = {0.f, 0.f};
                     for (int i = 0; i < trial_steps; i++)
                         for (int ph = 0; ph < 2; ph++) {
-                            BATCH_HIP(hipEventRecord(e0, stream), "hipEventRecord");
-                            BATCH_HIP(launch_phase(b, ph, stream), "batch kernel launch");
-                            BATCH_HIP(hipEventRecord(e1, stream), "hipEventRecord");
-                            BATCH_HIP(hipEventSynchronize(e1), "hipEventSynchronize");
                             float t = 0.f;
-                            BATCH_HIP(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
+                            if ((rc = timed([&] { return launch_phase(b, ph, stream); }, &t)))
+                                return rc;
                             if (i > 0)
                                 ms[ph] += t;
                         }
@@ -546,10 +711,37 @@ int lbm_batch_autotune(lbm_batch* b, int trial_steps, hipStream_t stream,
                             best.options[ph] = opt;
                         }
                 }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     // block / cells / grid shape are shared by both phases: keep alpha's, which
     // carries more arithmetic; beta keeps its own launch options.
+    float best_step = best_ms[0] + best_ms[1];
+    // the pair pass, when the members allow it (one pass = one whole step)
+    bool eligible = true;
+    for (int shape = 1; shape <= kPairShapes && eligible; shape++)
+    for (int opt = 0; opt < 4; opt++) {
+        cand = best;
+        cand.pairs = shape;
+        cand.options[0] = opt;
+        if (apply_tuning(b, cand) != LBM_OK) {
+            eligible = false;   // not eligible: keep the in-place launches
+            break;
+        }
+        float total = 0.f;
+        const int passes = 2 * trial_steps;
+        for (int i = 0; i < passes; i++) {
+            float t = 0.f;
+            if ((rc = timed([&] { return launch_pair(b, i & 1, stream); }, &t)))
+                return rc;
+            if (i >= 2)
+                total += t;
+        }
+        const float per_step = total / (float)(passes - 2) * (float)(trial_steps - 1);
+        if (per_step < best_step) {
+            best_step = per_step;
+            best = cand;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     if ((rc = apply_tuning(b, best)))
         return rc;
     if (chosen)

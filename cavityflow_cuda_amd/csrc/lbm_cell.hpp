@@ -141,13 +141,13 @@ __device__ __forceinline__ void xpack_beta(const StepArgs<typename P::C>& a, uin
 
 // One cell of CLbmAlphaCPU::alphaKernelCPU (CLbmAlphaCPU.cpp:75-494). The
 // reference's `tmp` is a #define for `rho` (:199): every gravity term
-// overwrites rho and the stored density is the last gravity term.
+// overwrites rho and the stored density is the last gravity term. `flag` is
+// the cell's flag, passed in so a caller that already holds it skips a load.
 template <class P, class A>
 __device__ __forceinline__ void alpha_cell(const StepArgs<typename P::C>& a, uint32_t id,
-                                           const A& acc)
+                                           int flag, const A& acc)
 {
     using C = typename P::C;
-    const int flag = a.flags[id];
     uint32_t row;
     bool pl, pr;
     const bool pack = xpack_where(a, id, row, pl, pr);
@@ -292,13 +292,13 @@ __device__ __forceinline__ void alpha_cell(const StepArgs<typename P::C>& a, uin
 // `dd_param` is a #define for `rho` (:224): the stored FLUID density is
 // rho - 1.5 |j|^2. OBSTACLE swaps opposite pairs, which writes back exactly
 // what was read, so its stores are skipped. GHOST_LAYER passes values through.
-// Neighbour indices wrap modulo the cell count (domainWrap, :44-50).
+// Neighbour indices wrap modulo the cell count (domainWrap, :44-50). `flag`
+// is the cell's flag, as for alpha_cell.
 template <class P, class A>
 __device__ __forceinline__ void beta_cell(const StepArgs<typename P::C>& a, uint32_t id,
-                                          const A& acc)
+                                          int flag, const A& acc)
 {
     using C = typename P::C;
-    const int flag = a.flags[id];
     uint32_t row;
     bool pl, pr;
     const bool pack = xpack_where(a, id, row, pl, pr);

@@ -87,6 +87,7 @@ struct lbm_solver {
     int residual_valid;
     int launch[2];     // LBM_LAUNCH_* options of the alpha and beta kernels
     void* xpack[2];    // fused x-face pack buffers (LEFT, RIGHT), NULL when off
+    void* dd_alt;      // second population buffer of the out-of-place pair pass
 };
 
 namespace lbm {

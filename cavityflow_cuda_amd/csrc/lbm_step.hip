@@ -115,10 +115,10 @@ __global__ __launch_bounds__(kBlock) void k_alpha(StepArgs<typename P::C> a)
     const uint32_t id = FULL ? gid : box_cell(a, gid);
     if (OFF32) {
         const Acc32<S, NT> acc{a.plane, id * (uint32_t)sizeof(S)};
-        alpha_cell<P>(a, id, acc);
+        alpha_cell<P>(a, id, a.flags[id], acc);
     } else {
         const Acc64<S, false> acc{(S*)a.dd, a.stride, (int32_t)id, a.n};
-        alpha_cell<P>(a, id, acc);
+        alpha_cell<P>(a, id, a.flags[id], acc);
     }
 }
 
@@ -141,13 +141,13 @@ __global__ __launch_bounds__(kBlock) void k_beta(StepArgs<typename P::C> a)
     const uint32_t id = FULL ? gid : box_cell(a, gid);
     if (wrap) {
         const Acc64<S, true> acc{(S*)a.dd, a.stride, (int32_t)id, a.n};
-        beta_cell<P>(a, id, acc);
+        beta_cell<P>(a, id, a.flags[id], acc);
     } else if (OFF32) {
         const Acc32<S, NT> acc{a.plane, id * (uint32_t)sizeof(S)};
-        beta_cell<P>(a, id, acc);
+        beta_cell<P>(a, id, a.flags[id], acc);
     } else {
         const Acc64<S, false> acc{(S*)a.dd, a.stride, (int32_t)id, a.n};
-        beta_cell<P>(a, id, acc);
+        beta_cell<P>(a, id, a.flags[id], acc);
     }
 }
 

@@ -10,14 +10,19 @@ Its CLbmSolver parameters come from the reference's time-step adaptation
 reference would exit) is rejected.
 
 ``CavityServer`` keeps every admitted cavity resident on the device as its own
-solver and steps all of them together: each scheduling round replays one
-captured graph of K whole steps over the current batch (one launch per
-half-step covering every member, ``lbm_batch_run``), then reads all residuals
-with one launch (``lbm_batch_residual``). Finished members leave and queued
-requests join between rounds -- every member is at an even half-step there,
-so the batch always runs alpha first. Admission is FIFO under an HBM budget:
-a request is admitted when its device footprint (``lbm_device_bytes``) fits
-in what the resident solves leave free; one that can never fit is rejected.
+solver and steps them in batches: each scheduling round replays one captured
+graph of K time steps over the current batch (one launch per half-step
+covering every member, ``lbm_batch_run``), then reads all its residuals with
+one launch (``lbm_batch_residual``). Finished members leave and queued
+requests join between rounds -- every member is at an even step there, so
+the batch always runs alpha first. Which resident solves a round advances is
+the policy: "srpt" takes them in order of least remaining work (cells x steps
+left to the cap) until the batch holds ``min_batch_cells`` cells, enough to
+fill the GPU, so short solves are not held behind long ones; "all" advances
+every resident solve every round (processor sharing). Admission is FIFO
+under an HBM budget: a request is admitted when its device footprint
+(``lbm_device_bytes``) fits in what the resident solves leave free; one that
+can never fit is rejected.
 
 ``solve_sequential`` is the plain baseline: requests one after another in
 arrival order, each on its own solver with one launch per half-step and the
@@ -136,7 +141,8 @@ class CavityServer:
     """Continuous-batching solve server on one GPU (see module docstring)."""
 
     def __init__(self, precision="f32", hbm_budget=200e9, check_every=100, device=0,
-                 stream=None, tuning=None, max_members=4096):
+                 stream=None, tuning=None, max_members=4096, policy="srpt",
+                 min_batch_cells=16_000_000):
         if check_every % 2 or check_every <= 0:
             raise ValueError("check_every must be even (whole alpha + beta pairs)")
         self.precision = precision
@@ -146,6 +152,11 @@ class CavityServer:
         self.stream = stream
         self.tuning = tuning
         self.max_members = int(max_members)
+        if policy not in ("srpt", "all"):
+            raise ValueError("policy is 'srpt' or 'all'")
+        self.policy = policy
+        self.min_batch_cells = int(min_batch_cells)
+        self._batch_ids: tuple = ()
         self.queue: deque = deque()
         self.active: list = []          # [req, solver, bytes, steps, admitted]
         self.used = 0
@@ -174,7 +185,8 @@ class CavityServer:
                 self._reject(req, str(e))
                 return
         try:
-            req._bytes = device_bytes(req.size, self.precision)
+            req._bytes = device_bytes(req.size, self.precision,
+                                      pair_buffer=bool(self.tuning and self.tuning.get("pairs")))
         except LbmError as e:
             self._reject(req, str(e))
             return
@@ -214,18 +226,32 @@ class CavityServer:
         if admitted:
             self._dirty = True
 
-    def _rebuild(self):
+    def _select(self) -> list:
+        """The resident solves this round advances."""
+        if self.policy == "all":
+            return list(self.active)
+        order = sorted(self.active, key=lambda a: (a[0].cells * (a[0].max_steps - a[3]), a[4]))
+        chosen, cells = [], 0
+        for a in order:
+            chosen.append(a)
+            cells += a[0].cells
+            if cells >= self.min_batch_cells:
+                break
+        return chosen
+
+    def _rebuild(self, members):
         if self._batch.value:
             check(lib().lbm_batch_destroy(self._batch))
             self._batch = ctypes.c_void_p()
-        if not self.active:
+        self._batch_ids = tuple(a[0].id for a in members)
+        if not members:
             return
-        arr = (ctypes.c_void_p * len(self.active))(*[a[1].handle.value for a in self.active])
-        check(lib().lbm_batch_create(arr, len(self.active), ctypes.byref(self._batch)))
+        arr = (ctypes.c_void_p * len(members))(*[a[1].handle.value for a in members])
+        check(lib().lbm_batch_create(arr, len(members), ctypes.byref(self._batch)))
         if self.tuning:
             t = self.tuning
             bt = _lib.lbm_batch_tuning(t["block"], t["cells_per_thread"], t["waves_per_cu"],
-                                       (ctypes.c_int * 2)(*t["options"]))
+                                       (ctypes.c_int * 2)(*t["options"]), t.get("pairs", 0))
             check(lib().lbm_batch_set_tuning(self._batch, ctypes.byref(bt)))
         self._dirty = False
 
@@ -256,15 +282,17 @@ class CavityServer:
             self._cancel.clear()
         if not self.active:
             return done
-        if self._dirty:
-            self._rebuild()
+        members = self._select()
+        if self._dirty or tuple(a[0].id for a in members) != self._batch_ids:
+            self._rebuild(members)
+            self._dirty = False
         st = _stream(self.stream)
         check(lib().lbm_batch_run(self._batch, 0, self.K, st))
-        res = (ctypes.c_double * len(self.active))()
+        res = (ctypes.c_double * len(members))()
         check(lib().lbm_batch_residual(self._batch, 0, res, st))
         self.rounds += 1
-        keep = []
-        for i, a in enumerate(self.active):
+        finished = set()
+        for i, a in enumerate(members):
             req, s = a[0], a[1]
             a[3] += self.K
             self.cell_updates += req.cells * self.K
@@ -277,19 +305,25 @@ class CavityServer:
             elif a[3] >= req.max_steps:
                 status = "step_cap"
             if status is None:
-                keep.append(a)
                 continue
             out = _finish(req, s, status, a[3], r, self.stream)
             out.admitted = a[4]
             out.finished = self.clock()
             if status == "failed":
                 out.message = "non-finite residual"
-            self._drop(a)
+            finished.add(req.id)
             self.results[out.id] = out
             done.append(out)
-        if len(keep) != len(self.active):
+        if finished:
+            # the batch goes before its members' arrays
+            check(lib().lbm_batch_destroy(self._batch))
+            self._batch = ctypes.c_void_p()
+            self._batch_ids = ()
+            for a in self.active:
+                if a[0].id in finished:
+                    self._drop(a)
+            self.active = [a for a in self.active if a[0].id not in finished]
             self._dirty = True
-        self.active = keep
         return done
 
     def _drop(self, a):

@@ -97,12 +97,13 @@ def make_params(size, bc=(_lib.OBSTACLE,) * 6, precision="f32", inv_tau=1.0, lid
     return p
 
 
-def device_bytes(size, precision="f32", store_outputs=False) -> int:
-    """Device bytes of one solver of this shape, residual snapshot included."""
+def device_bytes(size, precision="f32", store_outputs=False, pair_buffer=False) -> int:
+    """Device bytes of one solver of this shape, residual snapshot included
+    (and the second population buffer of the batch pair pass)."""
     p = make_params(size, precision=precision, store_densities=store_outputs,
                     store_velocities=store_outputs)
     b = ctypes.c_uint64()
-    check(lib().lbm_device_bytes(ctypes.byref(p), ctypes.byref(b)))
+    check(lib().lbm_device_bytes(ctypes.byref(p), int(bool(pair_buffer)), ctypes.byref(b)))
     return int(b.value)
 
 

@@ -304,6 +304,14 @@ typedef struct lbm_batch_tuning {
     int waves_per_cu;      /* 0: one workgroup per tile; k > 0: persistent grid of
                               k workgroups per CU striding over the tiles */
     int options[2];        /* LBM_LAUNCH_* for the alpha and beta launches */
+    int pairs;             /* 0: half-step launches. 1..4: one out-of-place pass per
+                              alpha + beta pair with column tile 32x8, 32x4, 16x8 or
+                              32x6 (two
+                              time steps per sweep over HBM, a second population
+                              buffer per member); needs F32 / F16S / BF16S, an
+                              all-OBSTACLE array boundary and no stored rho / u
+                              outputs. options[0] applies to the pass
+                              (NONTEMPORAL: streaming stores of the new state). */
 } lbm_batch_tuning;
 
 int lbm_batch_create(lbm_solver* const* members, int count, lbm_batch** out);
@@ -315,10 +323,13 @@ int lbm_batch_get_tuning(const lbm_batch* batch, lbm_batch_tuning* tuning);
 int lbm_batch_step(lbm_batch* batch, int phase, hipStream_t stream);
 /* half_steps alternating half-steps starting with first_phase. On a non-NULL
  * stream the sequence is captured once into a HIP graph (per first_phase,
- * half_steps and stream) and replayed on later calls. */
+ * half_steps and stream) and replayed on later calls. With tuning.pairs,
+ * first_phase must be 0 and half_steps a multiple of 4 (the state returns
+ * to each member's own buffer). */
 int lbm_batch_run(lbm_batch* batch, int first_phase, int half_steps, hipStream_t stream);
 /* Times every candidate shape (block, cells_per_thread, waves_per_cu,
- * launch options) for trial_steps whole steps each and keeps the fastest.
+ * launch options, and the pair pass when the members allow it) for
+ * trial_steps whole steps each and keeps the fastest.
  * The trials are real steps: the members advance; reset them afterwards
  * (lbm_reset) when their state matters. Synchronous. */
 int lbm_batch_autotune(lbm_batch* batch, int trial_steps, hipStream_t stream,
@@ -333,8 +344,9 @@ int lbm_reset(lbm_solver* solver);
 /* Device bytes lbm_create allocates for these parameters, plus the residual
  * snapshot lbm_residual / lbm_batch_residual add on first use (no
  * counterpart in the reference; used for HBM-budget admission). Validates
- * params like lbm_create; touches no device. */
-int lbm_device_bytes(const lbm_params* params, uint64_t* bytes);
+ * params like lbm_create; touches no device. pair_buffer = 1 adds the second
+ * population buffer of the batch pair pass. */
+int lbm_device_bytes(const lbm_params* params, int pair_buffer, uint64_t* bytes);
 
 /* ---- misc ---------------------------------------------------------------- */
 

@@ -26,7 +26,11 @@ def alone(hip, sp, precision, steps):
 TUNINGS = [dict(block=256, cells_per_thread=1, waves_per_cu=0, options=(1, 2)),
            dict(block=512, cells_per_thread=2, waves_per_cu=4, options=(0, 0)),
            dict(block=256, cells_per_thread=4, waves_per_cu=0, options=(3, 3)),
-           dict(block=512, cells_per_thread=1, waves_per_cu=16, options=(2, 1))]
+           dict(block=512, cells_per_thread=1, waves_per_cu=16, options=(2, 1)),
+           dict(block=256, cells_per_thread=1, waves_per_cu=0, options=(0, 0), pairs=1),
+           dict(block=256, cells_per_thread=1, waves_per_cu=0, options=(2, 0), pairs=2),
+           dict(block=256, cells_per_thread=1, waves_per_cu=0, options=(1, 0), pairs=3),
+           dict(block=256, cells_per_thread=1, waves_per_cu=0, options=(3, 0), pairs=4)]
 
 
 @pytest.mark.parametrize("precision", ["f32", "f64", "f16s", "bf16s"])
@@ -34,7 +38,9 @@ TUNINGS = [dict(block=256, cells_per_thread=1, waves_per_cu=0, options=(1, 2)),
 def test_batch_members_equal_solo_runs(hip, precision, tuning):
     from cavityflow_cuda_amd.batch import CavityBatch
     print("[case] batch", precision, TUNINGS[tuning], flush=True)
-    steps = 38
+    steps = 40
+    if precision == "f64" and TUNINGS[tuning].get("pairs"):
+        pytest.skip("the pair pass has no FP64 variant (covered by the rejection test)")
     sp = specs(hip, precision)
     b = CavityBatch(sp, precision)
     b.set_tuning(**TUNINGS[tuning])
@@ -104,3 +110,33 @@ def test_batch_residual_matches_solver_residual(hip):
     assert np.allclose(r1[finite], r2[finite], rtol=1e-9, atol=0)
     b.close()
     c.close()
+
+
+def test_pair_pass_rejections(hip):
+    from cavityflow_cuda_amd import LbmError, GHOST_LAYER, LbmSolverHIP
+    from cavityflow_cuda_amd.batch import CavityBatch
+    from cavityflow_cuda_amd import _lib
+    import ctypes
+    print("[case] batch pair rejections", flush=True)
+    b = CavityBatch(specs(hip, "f64"), "f64")
+    with pytest.raises(LbmError):
+        b.set_tuning(pairs=1)
+    b.close()
+    # a GHOST_LAYER face (a subdomain of a split cavity) is not a whole domain
+    s = LbmSolverHIP((8, 8, 8), (GHOST_LAYER,) + (_lib.OBSTACLE,) * 5, "f32", 1.5, 0.05)
+    h = ctypes.c_void_p()
+    arr = (ctypes.c_void_p * 1)(s.handle.value)
+    from cavityflow_cuda_amd._lib import check, lib
+    check(lib().lbm_batch_create(arr, 1, ctypes.byref(h)))
+    t = _lib.lbm_batch_tuning(256, 1, 0, (ctypes.c_int * 2)(0, 0), 1)
+    assert lib().lbm_batch_set_tuning(h, ctypes.byref(t)) == _lib.LBM_ERR_UNSUPPORTED
+    t.pairs = 0
+    check(lib().lbm_batch_set_tuning(h, ctypes.byref(t)))
+    # the pair pass needs multiples of 4 half-steps from phase 0
+    b2 = CavityBatch(specs(hip, "f32"), "f32")
+    b2.set_tuning(pairs=1)
+    assert lib().lbm_batch_run(b2._h, 0, 6, None) == _lib.LBM_ERR_ARG
+    assert lib().lbm_batch_run(b2._h, 1, 8, None) == _lib.LBM_ERR_ARG
+    b2.close()
+    lib().lbm_batch_destroy(h)
+    s.close()

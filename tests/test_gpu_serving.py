@@ -28,14 +28,19 @@ def serve_staggered(srv, reqs):
     return srv.results
 
 
+PAIRS = dict(block=256, cells_per_thread=1, waves_per_cu=0, options=(2, 0), pairs=1)
+
+
+@pytest.mark.parametrize("pairs", [0, 1])
 @pytest.mark.parametrize("precision", ["f32", "f16s"])
-def test_results_same_alone_and_in_the_mix(hip, precision):
+def test_results_same_alone_and_in_the_mix(hip, precision, pairs):
     from cavityflow_cuda_amd.serving import CavityServer, solve_sequential
-    print("[case] serving mix", precision, flush=True)
-    # tol 2e-4 lets the small, viscous cavities converge early, the others hit the cap
-    srv = CavityServer(precision, check_every=50)
-    mixed = serve_staggered(srv, requests(tol=2e-4))
-    alone = solve_sequential(requests(tol=2e-4), precision, check_every=50)
+    print("[case] serving mix", precision, "pairs", pairs, flush=True)
+    # tol 2e-4 lets the small, viscous cavities converge early, the others hit the cap;
+    # the solo runs always use the in-place half-step launches
+    srv = CavityServer(precision, check_every=52, tuning=PAIRS if pairs else None)
+    mixed = serve_staggered(srv, requests(tol=2e-4, max_steps=624))
+    alone = solve_sequential(requests(tol=2e-4, max_steps=624), precision, check_every=52)
     statuses = set()
     for i in range(len(MIX)):
         a, b = mixed[i], alone[i]
@@ -105,4 +110,27 @@ def test_open_loop_run_reports_latency(hip):
     assert s["completed"] == len(MIX)
     assert 0 < s["p50_s"] <= s["p99_s"] <= s["max_s"]
     assert all(r.admitted >= r.arrival for r in res.values())
+    srv.close()
+
+
+def test_srpt_runs_least_remaining_work_first(hip):
+    from cavityflow_cuda_amd.serving import CavityServer, SolveRequest, solve_sequential
+    print("[case] serving srpt order", flush=True)
+    reqs = [SolveRequest(0, (30, 30, 30), 0.1, 300.0, max_steps=400, keep_fields=True),
+            SolveRequest(1, (12, 10, 8), 0.1, 100.0, max_steps=400, keep_fields=True),
+            SolveRequest(2, (20, 16, 12), 0.1, 200.0, max_steps=200, keep_fields=True)]
+    # one member per round: always the one with the least cells x steps left
+    srv = CavityServer("f32", policy="srpt", min_batch_cells=1)
+    for r in reqs:
+        srv.submit(r)
+    order = []
+    while not srv.idle():
+        for r in srv.round():
+            order.append(r.id)
+    assert order == [1, 2, 0]
+    alone = solve_sequential([SolveRequest(r.id, r.size, r.u_lid, r.reynolds, max_steps=r.max_steps,
+                                           keep_fields=True) for r in reqs])
+    for i in range(3):
+        assert srv.results[i].steps == alone[i].steps
+        assert srv.results[i].fields.tobytes() == alone[i].fields.tobytes(), i
     srv.close()

@@ -27,6 +27,8 @@ def test_device_bytes_matches_allocation_formula(hip):
     partials = (1024 + 1) * 5 * 8
     assert device_bytes((32, 32, 32), "f32") == 19 * stride * 4 + stride + partials + 12 * stride
     assert device_bytes((32, 32, 32), "f16s") == 19 * stride * 2 + stride + partials + 12 * stride
+    assert (device_bytes((32, 32, 32), "f32", pair_buffer=True)
+            == device_bytes((32, 32, 32), "f32") + 19 * stride * 4)
     assert device_bytes((32, 32, 32), "f64", True) == (19 * stride * 8 + stride + partials
                                                       + 8 * stride + 24 * stride + 24 * stride)
     from cavityflow_cuda_amd import LbmError
