@@ -31,7 +31,8 @@ EXPORTED = (
     "lbm_set_launch_options", "lbm_get_launch_options", "lbm_set_step_pack", "lbm_get_moments",
     "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_size", "lbm_batch_set_tuning",
     "lbm_batch_get_tuning", "lbm_batch_step", "lbm_batch_run", "lbm_batch_autotune",
-    "lbm_batch_residual", "lbm_reset", "lbm_device_bytes",
+    "lbm_batch_residual", "lbm_reset", "lbm_device_bytes", "lbm_state_bytes",
+    "lbm_save_state", "lbm_load_state",
     "lbm_reduce_box", "lbm_residual", "lbm_sync", "lbm_last_error", "lbm_status_string",
     "lbm_abi_version",
 )
@@ -163,6 +164,9 @@ def lib() -> ctypes.CDLL:
         "lbm_batch_autotune": ([P, I, P, ctypes.POINTER(lbm_batch_tuning)], I),
         "lbm_batch_residual": ([P, I, ctypes.POINTER(ctypes.c_double), P], I),
         "lbm_reset": ([P], I),
+        "lbm_state_bytes": ([P, ctypes.POINTER(ctypes.c_uint64)], I),
+        "lbm_save_state": ([P, P, ctypes.c_uint64], I),
+        "lbm_load_state": ([P, P, ctypes.c_uint64], I),
         "lbm_device_bytes": ([ctypes.POINTER(lbm_params), I, ctypes.POINTER(ctypes.c_uint64)], I),
         "lbm_reduce_box": ([P, i3, i3, ctypes.POINTER(ctypes.c_double), P], I),
         "lbm_residual": ([P, I, ctypes.POINTER(ctypes.c_double), P], I),

@@ -874,6 +874,76 @@ int lbm_reset(lbm_solver* s)
     return LBM_OK;
 }
 
+int lbm_state_bytes(const lbm_solver* s, uint64_t* bytes)
+{
+    if (!s || !bytes) {
+        set_error("lbm_state_bytes: NULL argument");
+        return LBM_ERR_ARG;
+    }
+    *bytes = (uint64_t)s->n * ((uint64_t)kQ * s->elem_bytes + 1);
+    return LBM_OK;
+}
+
+namespace {
+
+// Copies between the padded device planes and a dense host state buffer.
+int state_copy(lbm_solver* s, void* host, uint64_t bytes, bool save)
+{
+    uint64_t need = 0;
+    int rc = lbm_state_bytes(s, &need);
+    if (rc)
+        return rc;
+    if (!host || bytes != need) {
+        set_error("state copy: buffer size does not match lbm_state_bytes");
+        return LBM_ERR_ARG;
+    }
+    if ((rc = set_device(s)))
+        return rc;
+    const size_t plane = (size_t)s->n * s->elem_bytes;
+    const size_t pitch = (size_t)s->stride * s->elem_bytes;
+    char* h = (char*)host;
+    char* d = (char*)s->dd;
+    if (save) {
+        LBM_HIP(hipMemcpy2D(h, plane, d, pitch, plane, kQ, hipMemcpyDeviceToHost), "D2H copy");
+        LBM_HIP(hipMemcpy(h + kQ * plane, s->flags, (size_t)s->n, hipMemcpyDeviceToHost),
+                "D2H copy");
+    } else {
+        LBM_HIP(hipMemcpy2D(d, pitch, h, plane, plane, kQ, hipMemcpyHostToDevice), "H2D copy");
+        LBM_HIP(hipMemcpy(s->flags, h + kQ * plane, (size_t)s->n, hipMemcpyHostToDevice),
+                "H2D copy");
+        s->residual_valid = 0;
+    }
+    return LBM_OK;
+}
+
+}  // namespace
+
+int lbm_save_state(lbm_solver* s, void* dst, uint64_t bytes)
+{
+    if (!s) {
+        set_error("lbm_save_state: solver is NULL");
+        return LBM_ERR_ARG;
+    }
+    int rc = set_device(s);
+    if (rc)
+        return rc;
+    LBM_HIP(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    return state_copy(s, dst, bytes, true);
+}
+
+int lbm_load_state(lbm_solver* s, const void* src, uint64_t bytes)
+{
+    if (!s) {
+        set_error("lbm_load_state: solver is NULL");
+        return LBM_ERR_ARG;
+    }
+    int rc = set_device(s);
+    if (rc)
+        return rc;
+    LBM_HIP(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    return state_copy(s, const_cast<void*>(src), bytes, false);
+}
+
 int lbm_sync(lbm_solver* s, hipStream_t stream)
 {
     if (!s) {

@@ -57,7 +57,9 @@ class SolveRequest:
     tol: float = 1e-6
     max_steps: int = 50000
     keep_fields: bool = False       # return the final [rho, jx, jy, jz] interior fields
-    initial: np.ndarray | None = None   # optional starting populations [19][z][y][x], interior
+    initial: np.ndarray | None = None   # optional start: LbmSolverHIP.save_state() (a
+    #                                     checkpoint, at an even step) or interior
+    #                                     populations [19][z][y][x]
     params: dict | None = None      # CLbmSolver-derived (filled on submit)
 
     @property
@@ -123,7 +125,10 @@ def _new_solver(req, precision, device):
                      p["velocity_dimless"][0], p["acceleration_dimless"], False, False, device)
     s.setDrivenCavitySzenario()
     if req.initial is not None:
-        s.setDensityDistributions(np.ascontiguousarray(req.initial, dtype=s.dtype).ravel())
+        if req.initial.dtype == np.uint8:
+            s.load_state(req.initial)
+        else:
+            s.setDensityDistributions(np.ascontiguousarray(req.initial, dtype=s.dtype).ravel())
     return s
 
 

@@ -271,6 +271,21 @@ class LbmSolverHIP:
     def sync(self, stream=None):
         check(lib().lbm_sync(self._h, _stream(stream)))
 
+    # -- checkpoint of the whole A-A state ---------------------------------
+    def save_state(self) -> np.ndarray:
+        """Populations of every cell (halo included, as stored) then flags,
+        as one uint8 array; restore with load_state on a solver of the same
+        size and precision."""
+        n = ctypes.c_uint64()
+        check(lib().lbm_state_bytes(self._h, ctypes.byref(n)))
+        out = np.empty(n.value, dtype=np.uint8)
+        check(lib().lbm_save_state(self._h, out.ctypes.data_as(ctypes.c_void_p), n.value))
+        return out
+
+    def load_state(self, state):
+        arr = np.ascontiguousarray(state, dtype=np.uint8).ravel()
+        check(lib().lbm_load_state(self._h, arr.ctypes.data_as(ctypes.c_void_p), arr.size))
+
     # -- CController::setDrivenCavitySzenario (CController.cpp:1313-1345) -----
     def setDrivenCavitySzenario(self):
         origin = (1, self.size_h[1] - 2, 1)

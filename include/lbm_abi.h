@@ -341,6 +341,19 @@ int lbm_batch_residual(lbm_batch* batch, int layout, double* residuals, hipStrea
  * (the state lbm_create leaves, lid included). Synchronous. */
 int lbm_reset(lbm_solver* solver);
 
+/* Checkpoint of the whole A-A state: the populations of every cell, halo
+ * included, as stored (F16S / BF16S keep their 16-bit codes, so a restore is
+ * exact), followed by the flags: [19][cells with halo] elements of the
+ * storage type, then [cells with halo] bytes. The argument-less
+ * get/setDensityDistributions of the reference cover the interior only
+ * (CLbmSolverCPU.cpp:552-559, 699-706), which loses the populations the A-A
+ * pattern keeps in the wall cells. No counterpart in the reference.
+ * lbm_state_bytes gives the size; save / load are synchronous host copies.
+ * A state saved after alpha must be continued with beta. */
+int lbm_state_bytes(const lbm_solver* solver, uint64_t* bytes);
+int lbm_save_state(lbm_solver* solver, void* dst, uint64_t bytes);
+int lbm_load_state(lbm_solver* solver, const void* src, uint64_t bytes);
+
 /* Device bytes lbm_create allocates for these parameters, plus the residual
  * snapshot lbm_residual / lbm_batch_residual add on first use (no
  * counterpart in the reference; used for HBM-budget admission). Validates
