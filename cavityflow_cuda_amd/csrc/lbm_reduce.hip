@@ -145,14 +145,20 @@ __global__ __launch_bounds__(kRedBlock) void k_moments(const typename P::S* __re
     dst[3 * (int64_t)a.cells + gid] = uz;
 }
 
+// Sum of the per-block partials in a fixed order (lane i adds blocks i, i + 64,
+// ... in turn, then the butterfly over the wave): the same bits on every call,
+// in a few microseconds instead of one lane walking all blocks.
 __global__ __launch_bounds__(64) void k_finalize(const double* partials, int nblocks,
                                                  double* out)
 {
-    if (threadIdx.x < kRedVals) {
+#pragma unroll
+    for (int k = 0; k < kRedVals; k++) {
         double t = 0.0;
-        for (int b = 0; b < nblocks; b++)
-            t += partials[b * kRedVals + threadIdx.x];
-        out[threadIdx.x] = t;
+        for (int b = threadIdx.x; b < nblocks; b += 64)
+            t += partials[b * kRedVals + k];
+        t = wave_sum(t);
+        if (threadIdx.x == 0)
+            out[k] = t;
     }
 }
 
