@@ -62,6 +62,8 @@ class CavityController:
         self.stream = stream
         self.simulationStepCounter = 0
         self.solvers = {}
+        self.writers = {}
+        self.config = None
         px, py, pz = self.split
         for k, j, i in itertools.product(range(pz), range(py), range(px)):
             bc = [_lib.GHOST_LAYER] * 6
@@ -83,6 +85,35 @@ class CavityController:
             if j == py - 1:
                 s.setDrivenCavitySzenario()
             self.solvers[(i, j, k)] = s
+
+    @classmethod
+    def from_config(cls, cfg, device=0, stream=None) -> "CavityController":
+        """The cavity an XML configuration describes (``config.LbmConfiguration``):
+        its size, subdomain split, CLbmSolver parameters and outputs, with a
+        VTK writer per rank when visualization is switched on (the reference
+        writes netCDF there, CController.cpp:120-135; netCDF is not part of
+        this build)."""
+        c = cls(cfg.domain_size, precision=cfg.precision, store_outputs=cfg.store_outputs,
+                split=cfg.subdomains, device=device, params=cfg.solver_parameters(),
+                stream=stream)
+        c.config = cfg
+        if cfg.do_visualization:
+            from .output import VtkWriter
+            px, py, _ = cfg.subdomains
+            for (i, j, k) in c.solvers:
+                rank = i + px * (j + py * k)
+                size, origin, length = cfg.subdomain(rank)
+                c.writers[(i, j, k)] = VtkWriter(rank, cfg.visualization_rate,
+                                                 cfg.visualization_dir, size, origin, length,
+                                                 cfg.precision)
+        return c
+
+    def render(self, iteration: int) -> list:
+        """CLbmVisualization::render of every rank (a file every `rate` iterations)."""
+        if self.writers:
+            self.sync()
+        return [p for key, w in self.writers.items()
+                if (p := w.render(iteration, self.solvers[key])) is not None]
 
     def close(self):
         for s in self.solvers.values():
@@ -129,10 +160,17 @@ class CavityController:
             break
 
     def run(self, loops: int) -> dict:
+        """CController::run (CController.cpp:1401-1440): render(0), then
+        each step followed by render(step counter) when writers are set."""
+        if loops < 0:
+            raise ValueError("loops must be >= 0 (the reference's endless run is not offered)")
+        self.render(self.simulationStepCounter)
         self.sync()
         t0 = time.perf_counter()
         for _ in range(loops):
             self.computeNextStep()
+            if self.writers:
+                self.render(self.simulationStepCounter)
         self.sync()
         elapsed = time.perf_counter() - t0
         cells = int(np.prod(self.size))

@@ -4,7 +4,18 @@
 //
 //   cavity_driver --size NX NY NZ [--split PX PY PZ] [--u-lid U] [--re RE]
 //                 [--loops N] [--precision f32|f64|f16s|bf16s] [--dump FILE]
+//   cavity_driver --config FILE.xml [--precision ...] [--loops N] [--dump FILE]
+//   cavity_driver --config FILE.xml --print-config
 //
+// * --config reads the reference's XML configuration (CConfiguration.cpp):
+//   size, subdomain split, physics, loops, outputs; the CLbmSolver parameters
+//   come from its physics for each rank's subdomain (CManager.cpp:28-58);
+//   with visualization on, every rank writes the VTK file of its interior
+//   at step 0 and after every step whose counter is a multiple of the rate
+//   (CController.cpp:1401-1430; the reference's active writer is netCDF,
+//   which this build does not have);
+// * --print-config prints the parsed fields and the derived parameters of
+//   rank 0 and exits without touching the GPU;
 // * subdomains are laid out like MPI ranks, x fastest (CManager.cpp:34-58),
 //   with OBSTACLE on global faces and GHOST_LAYER between subdomains (:91-104);
 //   the lid goes on the top layer of the top subdomains (:206-208);
@@ -25,6 +36,8 @@
 #include <vector>
 
 #include "CLbmSolverHIP.hpp"
+#include "lbm_config.hpp"
+#include "lbm_vtk.hpp"
 
 namespace {
 
@@ -35,6 +48,9 @@ struct Options {
     int loops = 100;
     std::string precision = "f32";
     std::string dump;
+    std::string config;
+    bool print_config = false;
+    bool loops_set = false;
 };
 
 int precision_code(const std::string& p)
@@ -46,28 +62,54 @@ int precision_code(const std::string& p)
     throw lbm::Error(LBM_ERR_ARG, "unknown precision " + p);
 }
 
+// CLbmSolver inputs: from the XML configuration (rank 0's subdomain, as every
+// rank derives the same cell length) or from --u-lid / --re with dx = dt = 1.
+lbm_physics physics(const Options& o, const lbm::Configuration* cfg)
+{
+    lbm_physics phys = {};
+    if (cfg) {
+        int size[3], origin[3];
+        double length[3];
+        cfg->subdomain(0, size, origin, length);
+        phys.length_x = length[0];
+        phys.size_x = size[0];
+        phys.length_max = std::max(cfg->domain_length[0],
+                                   std::max(cfg->domain_length[1], cfg->domain_length[2]));
+        for (int a = 0; a < 3; a++) {
+            phys.velocity[a] = cfg->velocity[a];
+            phys.acceleration[a] = cfg->acceleration[a];
+        }
+        phys.viscosity = cfg->viscosity;
+        phys.timestep = cfg->timestep;
+        phys.max_velocity = cfg->max_velocity;
+        phys.max_acceleration = cfg->max_acceleration;
+        return phys;
+    }
+    const int nmax = std::max(o.size[0], std::max(o.size[1], o.size[2]));
+    phys.length_x = o.size[0];
+    phys.size_x = o.size[0];
+    phys.length_max = nmax;
+    phys.velocity[0] = o.u_lid;
+    phys.viscosity = o.u_lid * nmax / o.re;
+    phys.timestep = 1.0;
+    phys.max_velocity = 0.1;
+    phys.max_acceleration = 0.1;
+    return phys;
+}
+
 template <typename T>
 class Controller {
 public:
-    Controller(const Options& o) : o_(o)
+    Controller(const Options& o, const lbm::Configuration* cfg) : o_(o), cfg_(cfg)
     {
         for (int a = 0; a < 3; a++) {
             if (o.size[a] % o.split[a])
                 throw lbm::Error(LBM_ERR_ARG, "domain size must be divisible by the split");
             sub_[a] = o.size[a] / o.split[a];
         }
-        const int nmax = std::max(o.size[0], std::max(o.size[1], o.size[2]));
-        lbm_physics phys = {};
-        phys.length_x = o.size[0];
-        phys.size_x = o.size[0];
-        phys.length_max = nmax;
-        phys.velocity[0] = o.u_lid;
-        phys.viscosity = o.u_lid * nmax / o.re;
-        phys.timestep = 1.0;
-        phys.max_velocity = 0.1;
-        phys.max_acceleration = 0.1;
         const int prec = precision_code(o.precision);
-        params_ = lbm::computeParameters(phys, prec);
+        params_ = lbm::computeParameters(physics(o, cfg), prec);
+        const bool outputs = cfg ? cfg->store_outputs() : true;
         for (int k = 0; k < o.split[2]; k++)
             for (int j = 0; j < o.split[1]; j++)
                 for (int i = 0; i < o.split[0]; i++) {
@@ -80,7 +122,8 @@ public:
                             bc[2 * a + 1] = lbm::OBSTACLE;
                     }
                     solvers_.emplace_back(new lbm::CLbmSolverHIP<T>(
-                        0, lbm::Vec3i(sub_[0], sub_[1], sub_[2]), bc, params_, true, true, prec));
+                        0, lbm::Vec3i(sub_[0], sub_[1], sub_[2]), bc, params_, outputs, outputs,
+                        prec));
                     if (j == o.split[1] - 1)
                         solvers_.back()->setDrivenCavitySzenario();
                 }
@@ -128,6 +171,33 @@ public:
 
     void sync() { lbm::check(lbm_sync(solvers_[0]->handle(), nullptr)); }
 
+    long counter() const { return counter_; }
+
+    // CLbmVisualization::render of every rank (CController.cpp:1401-1430)
+    void render(long iteration)
+    {
+        if (!cfg_ || !cfg_->do_visualization || iteration % cfg_->visualization_rate)
+            return;
+        const long sc = (long)sub_[0] * sub_[1] * sub_[2];
+        std::vector<T> rho(sc), u(3 * sc);
+        std::vector<lbm::Flag> flags(sc);
+        std::vector<int32_t> flag_values(sc);
+        for (int rank = 0; rank < (int)solvers_.size(); rank++) {
+            int size[3], origin[3];
+            double length[3];
+            cfg_->subdomain(rank, size, origin, length);
+            const T len[3] = {(T)length[0], (T)length[1], (T)length[2]};
+            auto* s = solvers_[rank].get();
+            s->getFlags(flags.data());
+            s->getDensities(rho.data());
+            s->getVelocities(u.data());
+            for (long c = 0; c < sc; c++)
+                flag_values[c] = flags[c];
+            lbm::write_vtk<T>(cfg_->visualization_dir, rank, (int)iteration, size, origin, len,
+                              flag_values.data(), rho.data(), u.data());
+        }
+    }
+
     void dump(const std::string& path)
     {
         const long nx = o_.size[0], ny = o_.size[1], nz = o_.size[2], cells = nx * ny * nz;
@@ -160,6 +230,7 @@ public:
 
 private:
     Options o_;
+    const lbm::Configuration* cfg_;
     int sub_[3];
     lbm::SolverParameters params_;
     std::vector<std::unique_ptr<lbm::CLbmSolverHIP<T>>> solvers_;
@@ -167,14 +238,17 @@ private:
 };
 
 template <typename T>
-int run(const Options& o)
+int run(const Options& o, const lbm::Configuration* cfg)
 {
-    Controller<T> c(o);
+    Controller<T> c(o, cfg);
     std::printf("tau:             %.17g\ninv tau:         %.17g\n", c.params().tau, c.params().tauInv);
+    c.render(0);
     c.sync();
     const auto t0 = std::chrono::steady_clock::now();
-    for (int i = 0; i < o.loops; i++)
+    for (int i = 0; i < o.loops; i++) {
         c.computeNextStep();
+        c.render(c.counter());
+    }
     c.sync();
     const double elapsed =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -187,6 +261,43 @@ int run(const Options& o)
     if (!o.dump.empty())
         c.dump(o.dump);
     return 0;
+}
+
+// Parsed fields (in the order velocity, acceleration, viscosity, max velocity /
+// acceleration, size, length, subdomains, CPU ratio, loops, timestep, four
+// switches, visualization rate, CPU blocks, GPU grids) and rank 0's
+// CLbmSolver parameters, one "key: values" line each, %.17g.
+void print_config(const lbm::Configuration& c, const lbm::SolverParameters& p)
+{
+    std::vector<double> v;
+    for (int a = 0; a < 3; a++) v.push_back(c.velocity[a]);
+    for (int a = 0; a < 3; a++) v.push_back(c.acceleration[a]);
+    v.push_back(c.viscosity);
+    v.push_back(c.max_velocity);
+    v.push_back(c.max_acceleration);
+    for (int a = 0; a < 3; a++) v.push_back(c.domain_size[a]);
+    for (int a = 0; a < 3; a++) v.push_back(c.domain_length[a]);
+    for (int a = 0; a < 3; a++) v.push_back(c.subdomains[a]);
+    v.push_back(c.cpu_subdomain_ratio);
+    v.push_back(c.loops);
+    v.push_back(c.timestep);
+    v.push_back(c.do_benchmark);
+    v.push_back(c.do_logging);
+    v.push_back(c.do_validation);
+    v.push_back(c.do_visualization);
+    v.push_back(c.visualization_rate);
+    for (auto& b : c.cpu_blocks)
+        for (int a = 0; a < 3; a++) v.push_back(b[a]);
+    for (auto& g : c.gpu_grids)
+        for (int a = 0; a < 3; a++) v.push_back(g[a]);
+    std::printf("fields:");
+    for (double x : v)
+        std::printf(" %.17g", x);
+    std::printf("\nvisualization-dir: %s\n", c.visualization_dir.c_str());
+    std::printf("params: %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n",
+                p.velocityDimLess[0], p.velocityDimLess[1], p.velocityDimLess[2],
+                p.accelerationDimLess[0], p.accelerationDimLess[1], p.accelerationDimLess[2],
+                p.viscosityDimLess, p.tau, p.tauInv, p.timestep, p.viscosity);
 }
 
 }  // namespace
@@ -217,6 +328,12 @@ int main(int argc, char** argv)
         } else if (a == "--loops") {
             need(1);
             o.loops = std::atoi(argv[++i]);
+            o.loops_set = true;
+        } else if (a == "--config") {
+            need(1);
+            o.config = argv[++i];
+        } else if (a == "--print-config") {
+            o.print_config = true;
         } else if (a == "--precision") {
             need(1);
             o.precision = argv[++i];
@@ -229,10 +346,32 @@ int main(int argc, char** argv)
         }
     }
     try {
+        std::unique_ptr<lbm::Configuration> cfg;
+        if (!o.config.empty()) {
+            cfg.reset(new lbm::Configuration(lbm::load_configuration(o.config, o.precision == "f64")));
+            for (int a = 0; a < 3; a++) {
+                o.size[a] = cfg->domain_size[a];
+                o.split[a] = cfg->subdomains[a];
+            }
+            if (!o.loops_set)
+                o.loops = cfg->loops;
+        }
+        if (o.print_config) {
+            if (!cfg)
+                throw lbm::Error(LBM_ERR_ARG, "--print-config needs --config");
+            print_config(*cfg, lbm::computeParameters(physics(o, cfg.get()),
+                                                      precision_code(o.precision)));
+            return 0;
+        }
+        if (o.loops < 0)
+            throw lbm::Error(LBM_ERR_ARG, "loops < 0 (an endless run) is not offered");
         if (o.precision == "f64")
-            return run<double>(o);
-        return run<float>(o);
+            return run<double>(o, cfg.get());
+        return run<float>(o, cfg.get());
     } catch (const lbm::Error& e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 1;
+    } catch (const std::runtime_error& e) {
         std::fprintf(stderr, "error: %s\n", e.what());
         return 1;
     }

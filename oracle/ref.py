@@ -38,6 +38,11 @@ def lib():
             getattr(_lib, f"ref_beta_{s}").argtypes = [_P, _P, _P, _P, ct, ct, _P, _P, _P, _P,
                                                       ctypes.c_int, ctypes.c_int]
             getattr(_lib, f"ref_solver_params_{s}").argtypes = [ctypes.c_char_p, _P]
+            getattr(_lib, f"ref_vtk_{s}").argtypes = [ctypes.c_char_p, ctypes.c_char_p,
+                                                     ctypes.c_int, ctypes.c_int, _P, _P, _P,
+                                                     _P, _P, _P]
+            getattr(_lib, f"ref_config_{s}").argtypes = [ctypes.c_char_p, _P, ctypes.c_char_p,
+                                                        ctypes.c_int]
     return _lib
 
 
@@ -101,3 +106,26 @@ def solver_params(xml_path: str, dtype=np.float32) -> np.ndarray:
     out = np.zeros(12, dtype=dtype)
     getattr(lib(), f"ref_solver_params_{_s(dtype)}")(xml_path.encode(), _ptr(out))
     return out
+
+
+def vtk(xml_path, out_dir, rank, iteration, size, origin, length, flags, rho, u, dtype):
+    """CLbmVisualizationVTK<T>::render of the given interior fields (flags
+    [z][y][x], rho [z][y][x], u [3][z][y][x]) into out_dir."""
+    dtype = np.dtype(dtype)
+    f = np.ascontiguousarray(flags, dtype=np.int32)
+    r = np.ascontiguousarray(rho, dtype=dtype)
+    v = np.ascontiguousarray(u, dtype=dtype)
+    ln = np.array(length, dtype=dtype)
+    getattr(lib(), f"ref_vtk_{_s(dtype)}")(xml_path.encode(), out_dir.encode(), int(rank),
+                                           int(iteration), _ptr(np.array(size, np.int32)),
+                                           _ptr(np.array(origin, np.int32)), _ptr(ln), _ptr(f),
+                                           _ptr(r), _ptr(v))
+
+
+def config(xml_path, dtype):
+    """Every field CConfiguration<T> reads (44 doubles, see ref_harness.cpp)
+    and the visualization output directory."""
+    out = np.zeros(44, dtype=np.float64)
+    buf = ctypes.create_string_buffer(4096)
+    getattr(lib(), f"ref_config_{_s(dtype)}")(xml_path.encode(), _ptr(out), buf, 4096)
+    return out, buf.value.decode()

@@ -4,9 +4,14 @@
 // under /root/reference/src (CLbm{Init,Alpha,Beta}CPU.cpp, CLbmSolver.cpp,
 // CConfiguration.cpp, CDomain.cpp, libmath, vendored tinyxml2) into
 // oracle/_ref/libref_lbm.so. It exists to (1) write the golden vectors under
-// tests/golden/ and (2) time the reference CPU path as the CPU baseline.
+// tests/golden/ (kernels, CLbmSolver parameters, CConfiguration fields, VTK
+// files of CLbmVisualizationVTK) and (2) time the reference CPU path as the
+// CPU baseline. Built with USE_MPI defined, as the reference's MPI build
+// (common.make:66-67), which only changes how CConfiguration reads
+// subdomain-num.
 // The product never loads it.
 #include <cstdint>
+#include <cstdio>
 #include <vector>
 
 #include "CConfiguration.hpp"
@@ -15,6 +20,7 @@
 #include "cpukernels/CLbmAlphaCPU.hpp"
 #include "cpukernels/CLbmBetaCPU.hpp"
 #include "cpukernels/CLbmInitCPU.hpp"
+#include "libvis/CLbmVisualizationVTK.hpp"
 
 static_assert(sizeof(Flag) == sizeof(int32_t), "reference Flag must be 32-bit");
 
@@ -89,6 +95,85 @@ struct SolverProbe : public CLbmSolver<T> {
     }
 };
 
+// Hands given interior flags / densities / velocities to the reference's VTK
+// writer (CLbmVisualizationVTK), which reads them through these getters.
+template <class T>
+struct VisProbe : public CLbmSolver<T> {
+    const int32_t* f;
+    const T* rho;
+    const T* u;
+    VisProbe(CDomain<T>& d, std::vector<Flag> bc, CConfiguration<T>* c, const int32_t* f,
+             const T* rho, const T* u)
+        : CLbmSolver<T>(0, d, bc, c), f(f), rho(rho), u(u)
+    {
+    }
+    int cells() { return this->getDomain()->getNumOfCells(); }
+    void getFlags(Flag* dst)
+    {
+        for (int i = 0; i < cells(); i++)
+            dst[i] = static_cast<Flag>(f[i]);
+    }
+    void getDensities(T* dst)
+    {
+        for (int i = 0; i < cells(); i++)
+            dst[i] = rho[i];
+    }
+    void getVelocities(T* dst)
+    {
+        for (int i = 0; i < 3 * cells(); i++)
+            dst[i] = u[i];
+    }
+};
+
+template <class T>
+void vtk(const char* xml_path, const char* dir, int id, int iteration, const int size[3],
+         const int origin[3], const T length[3], const int32_t* flags, const T* rho, const T* u)
+{
+    CConfiguration<T> cfg(xml_path);
+    CDomain<T> dom(id, v3(size), v3(origin), CVector<3, T>(length[0], length[1], length[2]));
+    std::vector<Flag> bc(6, OBSTACLE);
+    VisProbe<T> p(dom, bc, &cfg, flags, rho, u);
+    CLbmVisualizationVTK<T> vis(id, 1, &p, dir);
+    vis.render(iteration);
+}
+
+// Every field CConfiguration reads from the XML file, as double, in this
+// order: velocity[3], acceleration[3], viscosity, max velocity, max
+// acceleration, domain size[3], domain length[3], subdomains[3], CPU ratio,
+// loops, timestep, do benchmark / logging / validation / visualization,
+// visualization rate, alpha / beta CPU blocks[3 + 3], init / alpha / beta GPU
+// grids[3 + 3 + 3]; then the visualization output directory.
+template <class T>
+void config(const char* xml_path, double out[44], char* vis_dir, int cap)
+{
+    CConfiguration<T> c(xml_path);
+    int k = 0;
+    for (int i = 0; i < 3; i++) out[k++] = c.velocity[i];
+    for (int i = 0; i < 3; i++) out[k++] = c.acceleration[i];
+    out[k++] = c.viscosity;
+    out[k++] = c.maxVelocityDimLess;
+    out[k++] = c.maxAccelerationDimLess;
+    for (int i = 0; i < 3; i++) out[k++] = c.domainSize[i];
+    for (int i = 0; i < 3; i++) out[k++] = c.domainLength[i];
+    for (int i = 0; i < 3; i++) out[k++] = c.numOfSubdomains[i];
+    out[k++] = c.CPUSubdomainRatio;
+    out[k++] = c.loops;
+    out[k++] = c.timestep;
+    out[k++] = c.doBenchmark;
+    out[k++] = c.doLogging;
+    out[k++] = c.doValidation;
+    out[k++] = c.doVisualization;
+    out[k++] = c.visualizationRate;
+    for (int b = 0; b < 2; b++)
+        for (int i = 0; i < 3; i++) out[k++] = c.elementsPerBlock[b][i];
+    for (int g = 0; g < 3; g++) {
+        out[k++] = c.threadsPerBlock[g].x;
+        out[k++] = c.threadsPerBlock[g].y;
+        out[k++] = c.threadsPerBlock[g].z;
+    }
+    snprintf(vis_dir, cap, "%s", c.visualizationOutputDir.c_str());
+}
+
 template <class T>
 void solver_params(const char* xml_path, T out[12])
 {
@@ -149,6 +234,26 @@ void ref_solver_params_d(const char* xml_path, double out[12])
 {
     solver_params<double>(xml_path, out);
 }
-int ref_build_id(void) { return 1; }
+void ref_vtk_f(const char* xml_path, const char* dir, int id, int iteration, const int size[3],
+               const int origin[3], const float length[3], const int32_t* flags,
+               const float* rho, const float* u)
+{
+    vtk<float>(xml_path, dir, id, iteration, size, origin, length, flags, rho, u);
+}
+void ref_vtk_d(const char* xml_path, const char* dir, int id, int iteration, const int size[3],
+               const int origin[3], const double length[3], const int32_t* flags,
+               const double* rho, const double* u)
+{
+    vtk<double>(xml_path, dir, id, iteration, size, origin, length, flags, rho, u);
+}
+void ref_config_f(const char* xml_path, double out[44], char* vis_dir, int cap)
+{
+    config<float>(xml_path, out, vis_dir, cap);
+}
+void ref_config_d(const char* xml_path, double out[44], char* vis_dir, int cap)
+{
+    config<double>(xml_path, out, vis_dir, cap);
+}
+int ref_build_id(void) { return 2; }
 
 }  // extern "C"

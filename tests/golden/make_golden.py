@@ -209,9 +209,103 @@ def params_fixture():
         json.dump(out, f, indent=1)
 
 
+QUIRKS_XML = """<?xml version="1.0"?>
+<lbm-configuration>
+  <physics>
+    <velocity><x> 0.05 </x><y>0</y><z>-0.0</z></velocity>
+    <acceleration><x>0.0</x><y>-1e-3xyz</y><z>0</z></acceleration>
+    <viscosity>0.0123456789012345678</viscosity>
+    <max-velocity>0.1</max-velocity>
+    <max-acceleration>0.1</max-acceleration>
+  </physics>
+  <grid>
+    <domain-size><x>24.9</x><y> 12</y><z>36abc</z></domain-size>
+    <domian-length><x>2.4</x><y>1.2</y><z>3.6</z></domian-length>
+    <subdomain-num><x>2</x><y>1</y><z>3</z></subdomain-num>
+    <cpu-subdomain-ratio>0.25</cpu-subdomain-ratio>
+  </grid>
+  <simulation>
+    <loops>-1</loops>
+    <timestep>0.001</timestep>
+    <benchmark><do>1</do><output-dir>bench_out</output-dir></benchmark>
+    <logging><do>0</do><output-dir>log_out</output-dir></logging>
+    <validation><do>1</do><output-dir>val_out</output-dir></validation>
+    <visualization><do>1</do><rate>7</rate><output-dir>vis out/dir</output-dir></visualization>
+  </simulation>
+  <cpu><block-configuration>
+    <alpha-block-configuration><x>16</x><y>8</y><z>4</z></alpha-block-configuration>
+    <beta-block-configuration><x>8</x><y>4</y><z>2</z></beta-block-configuration>
+  </block-configuration></cpu>
+  <gpu><grid-configuration>
+    <init-grid-configuration><x>64</x><y>4</y><z>1</z></init-grid-configuration>
+    <alpha-grid-configuration><x>128</x><y>2</y><z>1</z></alpha-grid-configuration>
+    <beta-grid-configuration><x>256</x><y>1</y><z>1</z></beta-grid-configuration>
+  </grid-configuration></gpu>
+</lbm-configuration>
+"""
+
+
+def configs_fixture():
+    """The XML files behind params.json plus one exercising atoi / atof
+    prefixes, whitespace and the MPI subdomain count, with every field
+    CConfiguration<T> reads from each (configs/configs.json)."""
+    d = os.path.join(HERE, "configs")
+    os.makedirs(d, exist_ok=True)
+    files = []
+    for i, case in enumerate(PARAM_CASES):
+        name = f"case{i}.xml"
+        with open(os.path.join(d, name), "w") as f:
+            f.write(xml_config(**case))
+        files.append(name)
+    with open(os.path.join(d, "quirks.xml"), "w") as f:
+        f.write(QUIRKS_XML)
+    files.append("quirks.xml")
+    out = {}
+    for name in files:
+        entry = {}
+        for dtype in (np.float32, np.float64):
+            vals, vis = ref.config(os.path.join(d, name), dtype)
+            entry[np.dtype(dtype).name] = dict(values=vals[:41].tolist(), visualization_dir=vis)
+        out[name] = entry
+    with open(os.path.join(d, "configs.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
+def vtk_fixture():
+    """CLbmVisualizationVTK output for given interior fields: FP32 on a first
+    subdomain, FP64 on an offset subdomain with non-unit lengths; values span
+    signs, zeros and exponents so the number formatting is pinned."""
+    d = os.path.join(HERE, "vtk")
+    os.makedirs(d, exist_ok=True)
+    rng = np.random.default_rng(SEED + 7)
+    cases = [("f32", np.float32, (3, 4, 2), (0, 0, 0), (3.0, 4.0, 2.0), 0, 0),
+             ("f64", np.float64, (5, 2, 3), (5, 0, 2), (1.0, 0.4, 0.6), 3, 120)]
+    arrays = {}
+    for tag, dtype, size, origin, length, rank, it in cases:
+        n = int(np.prod(size))
+        flags = rng.choice(np.array([1, 2, 4, 8], np.int32), n)
+        rho = (rng.standard_normal(n) * 10.0 ** rng.integers(-12, 12, n)).astype(dtype)
+        u = (rng.standard_normal(3 * n) * 10.0 ** rng.integers(-20, 3, 3 * n)).astype(dtype)
+        rho[:3] = [0.0, -0.0, 1.0]
+        u[:4] = [0.0, -0.0, 123456789.0, 1e-5]
+        ref.vtk(os.path.join(HERE, "configs", "case0.xml"), d, rank, it, size, origin, length,
+                flags, rho, u, dtype)
+        arrays.update({f"{tag}_flags": flags, f"{tag}_rho": rho, f"{tag}_u": u,
+                       f"{tag}_size": np.array(size), f"{tag}_origin": np.array(origin),
+                       f"{tag}_length": np.array(length, dtype=dtype),
+                       f"{tag}_rank_iteration": np.array([rank, it])})
+    np.savez_compressed(os.path.join(d, "inputs.npz"), **arrays)
+
+
 def main():
     if not ref.available():
         raise SystemExit("oracle/_ref/libref_lbm.so missing: run `make -C oracle ref` first")
+    if len(sys.argv) > 1:             # only the named fixtures, e.g. `configs vtk`
+        for name in sys.argv[1:]:
+            globals()[f"{name}_fixture"]()
+        return
+    configs_fixture()
+    vtk_fixture()
     step_fixture(np.float32, "step_f32.npz")
     step_fixture(np.float64, "step_f64.npz")
     flags_fixture()
