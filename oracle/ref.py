@@ -15,6 +15,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_ref", "libref_lbm.so")
+REF_ROOT = os.environ.get("REF", "/root/reference")   # the reference tree (fixture source)
 
 _lib = None
 _i3 = ctypes.c_int * 3

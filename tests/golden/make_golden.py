@@ -260,6 +260,15 @@ def configs_fixture():
     with open(os.path.join(d, "quirks.xml"), "w") as f:
         f.write(QUIRKS_XML)
     files.append("quirks.xml")
+    # the configuration files the reference ships (data, copied as fixtures)
+    shipped = os.path.join(ref.REF_ROOT, "configurations")
+    for name in sorted(os.listdir(shipped)):
+        if name.endswith(".xml"):
+            with open(os.path.join(shipped, name)) as f:
+                text = f.read()
+            with open(os.path.join(d, "reference_" + name), "w") as f:
+                f.write(text)
+            files.append("reference_" + name)
     out = {}
     for name in files:
         entry = {}
