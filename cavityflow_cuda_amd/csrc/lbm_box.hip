@@ -30,7 +30,9 @@ __device__ __forceinline__ uint32_t box_index(const BoxArgs& a, uint32_t gid)
 }
 
 // populations: grid.y = lattice vector; shifted storage is converted to f
-template <class P>
+// NT: streaming accesses, for boxes one cell thick on some axis (the faces of
+// a halo exchange), which the next step does not reuse from the caches.
+template <class P, bool NT>
 __global__ __launch_bounds__(kCopyBlock) void k_get_dd(const typename P::S* __restrict__ dd,
                                                        typename P::C* __restrict__ dst,
                                                        BoxArgs a)
@@ -40,10 +42,12 @@ __global__ __launch_bounds__(kCopyBlock) void k_get_dd(const typename P::S* __re
     if (gid >= a.cells)
         return;
     const uint32_t id = box_index(a, gid);
-    dst[(int64_t)l * a.cells + gid] = P::load(dd[(int64_t)l * a.stride + id], l);
+    const typename P::S v = NT ? __builtin_nontemporal_load(dd + (int64_t)l * a.stride + id)
+                               : dd[(int64_t)l * a.stride + id];
+    dst[(int64_t)l * a.cells + gid] = P::load(v, l);
 }
 
-template <class P>
+template <class P, bool NT>
 __global__ __launch_bounds__(kCopyBlock) void k_set_dd(typename P::S* __restrict__ dd,
                                                        const typename P::C* __restrict__ src,
                                                        BoxArgs a)
@@ -55,7 +59,11 @@ __global__ __launch_bounds__(kCopyBlock) void k_set_dd(typename P::S* __restrict
     if (gid >= a.cells)
         return;
     const uint32_t id = box_index(a, gid);
-    dd[(int64_t)l * a.stride + id] = P::store(src[(int64_t)l * a.cells + gid], l);
+    const typename P::S v = P::store(src[(int64_t)l * a.cells + gid], l);
+    if (NT)
+        __builtin_nontemporal_store(v, dd + (int64_t)l * a.stride + id);
+    else
+        dd[(int64_t)l * a.stride + id] = v;
 }
 
 // rho / u fields (compute type, no conversion)
@@ -121,8 +129,12 @@ static hipError_t get_dd_t(const lbm_solver* s, const Box& b, void* dst, hipStre
 {
     BoxArgs a = box_args(s, b, 0x7FFFFu);
     dim3 grid((a.cells + kCopyBlock - 1) / kCopyBlock, kQ);
-    hipLaunchKernelGGL(k_get_dd<P>, grid, dim3(kCopyBlock), 0, st,
-                       (const typename P::S*)s->dd, (typename P::C*)dst, a);
+    if (b.bx == 1 || b.by == 1 || b.bz == 1)
+        hipLaunchKernelGGL((k_get_dd<P, true>), grid, dim3(kCopyBlock), 0, st,
+                           (const typename P::S*)s->dd, (typename P::C*)dst, a);
+    else
+        hipLaunchKernelGGL((k_get_dd<P, false>), grid, dim3(kCopyBlock), 0, st,
+                           (const typename P::S*)s->dd, (typename P::C*)dst, a);
     return hipGetLastError();
 }
 
@@ -132,8 +144,12 @@ static hipError_t set_dd_t(const lbm_solver* s, const Box& b, const void* src, u
 {
     BoxArgs a = box_args(s, b, mask);
     dim3 grid((a.cells + kCopyBlock - 1) / kCopyBlock, kQ);
-    hipLaunchKernelGGL(k_set_dd<P>, grid, dim3(kCopyBlock), 0, st, (typename P::S*)s->dd,
-                       (const typename P::C*)src, a);
+    if (b.bx == 1 || b.by == 1 || b.bz == 1)
+        hipLaunchKernelGGL((k_set_dd<P, true>), grid, dim3(kCopyBlock), 0, st,
+                           (typename P::S*)s->dd, (const typename P::C*)src, a);
+    else
+        hipLaunchKernelGGL((k_set_dd<P, false>), grid, dim3(kCopyBlock), 0, st,
+                           (typename P::S*)s->dd, (const typename P::C*)src, a);
     return hipGetLastError();
 }
 
