@@ -18,8 +18,10 @@ requests join between rounds -- every member is at an even step there, so
 the batch always runs alpha first. Which resident solves a round advances is
 the policy: "srpt" takes them in order of least remaining work (cells x steps
 left to the cap) until the batch holds ``min_batch_cells`` cells, enough to
-fill the GPU, so short solves are not held behind long ones; "all" advances
-every resident solve every round (processor sharing). Admission is FIFO
+fill the GPU, so short solves are not held behind long ones; "srpt_oldest"
+adds the longest-resident solve to that batch every round, so no solve
+waits behind an endless stream of shorter ones; "all" advances every
+resident solve every round (processor sharing). Admission is FIFO
 under an HBM budget: a request is admitted when its device footprint
 (``lbm_device_bytes``) fits in what the resident solves leave free; one that
 can never fit is rejected.
@@ -146,7 +148,7 @@ class CavityServer:
     """Continuous-batching solve server on one GPU (see module docstring)."""
 
     def __init__(self, precision="f32", hbm_budget=200e9, check_every=100, device=0,
-                 stream=None, tuning=None, max_members=4096, policy="srpt",
+                 stream=None, tuning=None, max_members=4096, policy="srpt_oldest",
                  min_batch_cells=16_000_000):
         if check_every % 2 or check_every <= 0:
             raise ValueError("check_every must be even (whole alpha + beta pairs)")
@@ -157,8 +159,8 @@ class CavityServer:
         self.stream = stream
         self.tuning = tuning
         self.max_members = int(max_members)
-        if policy not in ("srpt", "all"):
-            raise ValueError("policy is 'srpt' or 'all'")
+        if policy not in ("srpt", "srpt_oldest", "all"):
+            raise ValueError("policy is 'srpt', 'srpt_oldest' or 'all'")
         self.policy = policy
         self.min_batch_cells = int(min_batch_cells)
         self._batch_ids: tuple = ()
@@ -242,6 +244,10 @@ class CavityServer:
             cells += a[0].cells
             if cells >= self.min_batch_cells:
                 break
+        if self.policy == "srpt_oldest":
+            oldest = min(self.active, key=lambda a: a[4])
+            if all(a is not oldest for a in chosen):
+                chosen.append(oldest)
         return chosen
 
     def _rebuild(self, members):

@@ -134,3 +134,23 @@ def test_srpt_runs_least_remaining_work_first(hip):
         assert srv.results[i].steps == alone[i].steps
         assert srv.results[i].fields.tobytes() == alone[i].fields.tobytes(), i
     srv.close()
+
+
+def test_srpt_oldest_keeps_the_oldest_solve_moving(hip):
+    from cavityflow_cuda_amd.serving import CavityServer, SolveRequest
+    print("[case] serving srpt_oldest", flush=True)
+    srv = CavityServer("f32", policy="srpt_oldest", min_batch_cells=1)
+    srv.submit(SolveRequest(0, (30, 30, 30), 0.1, 300.0, max_steps=400))
+    srv.round()                              # request 0 is the oldest resident solve
+    for i in range(1, 4):
+        srv.submit(SolveRequest(i, (10, 10, 10), 0.1, 100.0, max_steps=200))
+    steps0 = []
+    while not srv.idle():
+        srv.round()
+        a0 = [a for a in srv.active if a[0].id == 0]
+        if a0:
+            steps0.append(a0[0][3])
+    # the big, oldest solve advanced every round while the small ones ran
+    assert steps0 == sorted(steps0) and len(set(steps0)) == len(steps0)
+    assert all(srv.results[i].status == "step_cap" for i in range(4))
+    srv.close()
