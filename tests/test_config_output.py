@@ -154,3 +154,30 @@ def test_native_driver_rejects_bad_configurations(tmp_path, hip):
         r = subprocess.run([exe, "--config", str(p), "--print-config"], capture_output=True,
                            text=True)
         assert r.returncode != 0 and r.stderr.startswith("error:")
+
+
+def test_native_driver_survives_malformed_xml(tmp_path, hip):
+    """Truncated, cut and garbage-injected files end in an error status,
+    never a crash (the reader was also run under ASan / UBSan)."""
+    import random
+    import subprocess
+    exe = _driver()
+    base = open(os.path.join(CONFIGS, "case0.xml")).read()
+    rng = random.Random(7)
+    for k in range(60):
+        s = base
+        if k % 3 == 0:
+            s = s[:rng.randrange(len(s))]
+        elif k % 3 == 1:
+            i = rng.randrange(len(s))
+            s = s[:i] + s[i + rng.randrange(1, 40):]
+        else:
+            i = rng.randrange(len(s))
+            s = s[:i] + rng.choice(["<", ">", "</", "<!--", "&", "<a b='1'>", "/>"]) + s[i:]
+        p = tmp_path / f"c{k}.xml"
+        p.write_text(s)
+        r = subprocess.run([exe, "--config", str(p), "--print-config"], capture_output=True,
+                           text=True, timeout=30)
+        assert r.returncode in (0, 1), (k, r.returncode, r.stderr[-300:])
+        if r.returncode:
+            assert r.stderr.startswith("error:"), r.stderr
