@@ -16,6 +16,13 @@
 //   which this build does not have);
 // * --print-config prints the parsed fields and the derived parameters of
 //   rank 0 and exits without touching the GPU;
+// * with validation on and more than one subdomain, the same configuration is
+//   run again as one subdomain and the velocities are compared cell by cell
+//   (main.cpp:121-303): a cell differs when |u - u_1| / u_1 > epsilon(T) on
+//   some component; the differing cells and their count go to
+//   <validation dir>/validation_<ranks>.txt (the reference appends the
+//   directory and name twice into one path, and indexes the gathered field
+//   with swapped strides, which only matters off the cube; both are fixed);
 // * subdomains are laid out like MPI ranks, x fastest (CManager.cpp:34-58),
 //   with OBSTACLE on global faces and GHOST_LAYER between subdomains (:91-104);
 //   the lid goes on the top layer of the top subdomains (:206-208);
@@ -30,8 +37,11 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <cmath>
 #include <fstream>
+#include <limits>
 #include <memory>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -198,10 +208,19 @@ public:
         }
     }
 
-    void dump(const std::string& path)
+    // interior velocities [3][z][y][x] of the whole cavity
+    std::vector<T> velocities()
+    {
+        std::vector<T> u, rho;
+        gather(u, rho);
+        return u;
+    }
+
+    void gather(std::vector<T>& u, std::vector<T>& rho)
     {
         const long nx = o_.size[0], ny = o_.size[1], nz = o_.size[2], cells = nx * ny * nz;
-        std::vector<T> u(3 * cells), rho(cells);
+        u.assign(3 * cells, T(0));
+        rho.assign(cells, T(0));
         const long sc = (long)sub_[0] * sub_[1] * sub_[2];
         std::vector<T> pu(3 * sc), pr(sc);
         for (int k = 0; k < o_.split[2]; k++)
@@ -221,6 +240,12 @@ public:
                                 rho[gid] = pr[lid];
                             }
                 }
+    }
+
+    void dump(const std::string& path)
+    {
+        std::vector<T> u, rho;
+        gather(u, rho);
         std::ofstream f(path, std::ios::binary);
         f.write(reinterpret_cast<const char*>(u.data()), u.size() * sizeof(T));
         f.write(reinterpret_cast<const char*>(rho.data()), rho.size() * sizeof(T));
@@ -236,6 +261,51 @@ private:
     std::vector<std::unique_ptr<lbm::CLbmSolverHIP<T>>> solvers_;
     long counter_ = 0;
 };
+
+// main.cpp:121-303: the configuration again on one subdomain (the
+// reference's serialize(); its visualization is not repeated here), then a
+// cell-by-cell comparison of the velocities.
+template <typename T>
+int validate(const Options& o, const lbm::Configuration& cfg, const std::vector<T>& u, int ranks)
+{
+    lbm::Configuration one = cfg;
+    for (int a = 0; a < 3; a++)
+        one.subdomains[a] = 1;
+    one.do_visualization = 0;
+    Options o1 = o;
+    for (int a = 0; a < 3; a++)
+        o1.split[a] = 1;
+    Controller<T> mono(o1, &one);
+    for (int i = 0; i < o.loops; i++)
+        mono.computeNextStep();
+    mono.sync();
+    const std::vector<T> v = mono.velocities();
+    const long nx = o.size[0], ny = o.size[1], nz = o.size[2], cells = nx * ny * nz;
+    std::stringstream name;
+    name << cfg.validation_dir << "/validation_" << ranks << ".txt";
+    std::ofstream f(name.str().c_str(), std::ios::out);
+    if (!f)
+        throw lbm::Error(LBM_ERR_ARG, "cannot open " + name.str());
+    const T eps = std::numeric_limits<T>::epsilon();
+    long unequal = 0;
+    for (long i = 0; i < nx; i++)
+        for (long j = 0; j < ny; j++)
+            for (long k = 0; k < nz; k++) {
+                const long g = (k * ny + j) * nx + i;
+                bool diff = false;
+                for (int c = 0; c < 3; c++)
+                    diff = diff || std::fabs(u[c * cells + g] - v[c * cells + g]) / v[c * cells + g] > eps;
+                if (!diff)
+                    continue;
+                f << "[" << i << "," << j << "," << k << "]: [" << u[g] << ", " << u[cells + g] << ", "
+                  << u[2 * cells + g] << "] vs. [" << v[g] << ", " << v[cells + g] << ", "
+                  << v[2 * cells + g] << "]" << std::endl;
+                unequal++;
+            }
+    f << "number of inequalities: " << unequal << std::endl;
+    std::printf("validation:      %ld inequalities (%s)\n", unequal, name.str().c_str());
+    return 0;
+}
 
 template <typename T>
 int run(const Options& o, const lbm::Configuration* cfg)
@@ -260,6 +330,9 @@ int run(const Options& o, const lbm::Configuration* cfg)
                 o.loops, elapsed, its, glups, glups * (19 * 2 * pop_bytes + 4));
     if (!o.dump.empty())
         c.dump(o.dump);
+    const int ranks = o.split[0] * o.split[1] * o.split[2];
+    if (cfg && cfg->do_validation && ranks > 1)
+        return validate<T>(o, *cfg, c.velocities(), ranks);
     return 0;
 }
 

@@ -57,3 +57,29 @@ def test_driver_and_controller_write_identical_vtk(tmp_path, hip, precision):
     text = (py_dir / "visualization_1_6.vtk").read_text().splitlines()
     assert text[4] == "DIMENSIONS 7 11 9" and text[6].split()[0] == "0.6"
     assert "nan" not in (py_dir / "visualization_0_6.vtk").read_text()
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_driver_validation_mode(tmp_path, hip, precision):
+    """validation do=1 (main.cpp:121-303): the split run is compared cell by
+    cell with a one-subdomain run of the same configuration."""
+    from cavityflow_cuda_amd import build
+    print("[case] validation", precision, flush=True)
+    exe = os.path.join(os.path.dirname(build.__file__), "cavity_driver")
+    vdir = tmp_path / "val"
+    vdir.mkdir()
+    text = config_text(str(tmp_path / "unused"))
+    text = text.replace("<visualization><do>1</do>", "<visualization><do>0</do>")
+    text = text.replace("<validation><do>0</do><output-dir>.</output-dir>",
+                        f"<validation><do>1</do><output-dir>{vdir}</output-dir>")
+    text = text.replace("<subdomain-num><x>2</x><y>1</y><z>1</z>",
+                        "<subdomain-num><x>2</x><y>2</y><z>1</z>")
+    assert str(vdir) in text and "<y>2</y><z>1</z></subdomain-num>" in text
+    cfg = tmp_path / "v.xml"
+    cfg.write_text(text)
+    r = subprocess.run([exe, "--config", str(cfg), "--precision", precision],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "validation:      0 inequalities" in r.stdout, r.stdout
+    lines = (vdir / "validation_4.txt").read_text().splitlines()
+    assert lines == ["number of inequalities: 0"]
