@@ -30,7 +30,10 @@
 //   exchanges faces after each half-step in the order x, y, z
 //   (CController.cpp:369-570, 807-1008), on the device;
 // * the benchmark figures are the reference's (CController.cpp:1382-1490):
-//   GLUPS over interior cells and GBytes/s = GLUPS * (19 * 2 * sizeof(T) + 4);
+//   GLUPS over interior cells and GBytes/s = GLUPS * bytes per cell (19 * 2
+//   populations + the flag + 4 outputs when stored); with benchmark on in
+//   the configuration every rank's figures also go to
+//   <benchmark dir>/benchmark_<ranks>_<rank>.txt;
 // * --dump writes the interior velocities [3][z][y][x] and densities [z][y][x]
 //   of the whole cavity (type T) to FILE.
 #include <chrono>
@@ -322,15 +325,35 @@ int run(const Options& o, const lbm::Configuration* cfg)
     c.sync();
     const double elapsed =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    const double its = o.loops / elapsed;
-    const double glups = its * o.size[0] * (double)o.size[1] * o.size[2] * 1e-9;
+    // CController.cpp:1380-1399, 1451-1466: bytes per cell = populations read
+    // and written, the flag, and the stored rho / u when there are outputs;
+    // the figures are computed in T and printed with stream defaults
     const int pop_bytes = o.precision == "f64" ? 8 : (o.precision == "f32" ? 4 : 2);
-    std::printf("loops:           %d\ntime:            %gs\niterations:      %gs^-1\n"
-                "lattice updates: %gGLUPS\nbandwidth:       %gGBytes/s\n",
-                o.loops, elapsed, its, glups, glups * (19 * 2 * pop_bytes + 4));
+    const bool outputs = cfg ? cfg->store_outputs() : true;
+    const int used = 19 * 2 * pop_bytes + 4 + (outputs ? 4 * (int)sizeof(T) : 0);
+    const T el = (T)elapsed;
+    const T its = (T)o.loops / el;
+    const T glups = its * (T)o.size[0] * (T)o.size[1] * (T)o.size[2] * (T)0.000000001;
+    const T bandwidth = glups * (T)used;
+    std::stringstream report;
+    report << "loops:           " << o.loops << std::endl
+           << "time:            " << el << "s" << std::endl
+           << "iterations:      " << its << "s^-1" << std::endl
+           << "lattice updates: " << glups << "GLUPS" << std::endl
+           << "bandwidth:       " << bandwidth << "GBytes/s" << std::endl;
+    std::fputs(report.str().c_str(), stdout);
+    const int ranks = o.split[0] * o.split[1] * o.split[2];
+    if (cfg && cfg->do_benchmark)
+        for (int rank = 0; rank < ranks; rank++) {
+            std::stringstream name;
+            name << cfg->benchmark_dir << "/benchmark_" << ranks << "_" << rank << ".txt";
+            std::ofstream f(name.str().c_str(), std::ios::out);
+            if (!f)
+                throw lbm::Error(LBM_ERR_ARG, "cannot open " + name.str());
+            f << report.str();
+        }
     if (!o.dump.empty())
         c.dump(o.dump);
-    const int ranks = o.split[0] * o.split[1] * o.split[2];
     if (cfg && cfg->do_validation && ranks > 1)
         return validate<T>(o, *cfg, c.velocities(), ranks);
     return 0;

@@ -83,3 +83,31 @@ def test_driver_validation_mode(tmp_path, hip, precision):
     assert "validation:      0 inequalities" in r.stdout, r.stdout
     lines = (vdir / "validation_4.txt").read_text().splitlines()
     assert lines == ["number of inequalities: 0"]
+
+
+def test_driver_benchmark_files(tmp_path, hip):
+    """benchmark do=1 (CController.cpp:1451-1490): every rank writes the
+    loops / time / iterations / GLUPS / GBytes/s lines, with bytes per cell
+    = 19 x 2 populations + flag + 4 stored outputs."""
+    from cavityflow_cuda_amd import build
+    print("[case] benchmark files", flush=True)
+    exe = os.path.join(os.path.dirname(build.__file__), "cavity_driver")
+    bdir = tmp_path / "bench"
+    bdir.mkdir()
+    text = config_text(str(tmp_path / "vis"))
+    (tmp_path / "vis").mkdir()
+    text = text.replace("<benchmark><do>0</do><output-dir>.</output-dir>",
+                        f"<benchmark><do>1</do><output-dir>{bdir}</output-dir>")
+    assert str(bdir) in text
+    cfg = tmp_path / "b.xml"
+    cfg.write_text(text)
+    r = subprocess.run([exe, "--config", str(cfg)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    report = [l for l in r.stdout.splitlines() if l.split(":")[0] in
+              ("loops", "time", "iterations", "lattice updates", "bandwidth")]
+    assert len(report) == 5 and report[0] == "loops:           6"
+    for rank in (0, 1):
+        assert (bdir / f"benchmark_2_{rank}.txt").read_text().splitlines() == report
+    glups = float(report[3].split()[-1].replace("GLUPS", ""))
+    gbs = float(report[4].split()[-1].replace("GBytes/s", ""))
+    assert abs(gbs / glups - (19 * 2 * 4 + 4 + 4 * 4)) < 1e-3 * 172
