@@ -265,13 +265,26 @@ int upload_tiles(lbm_batch* b, uint32_t tile_cells, std::vector<MemberDesc>& des
     return LBM_OK;
 }
 
+constexpr int kBatchOptions =
+    LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_XCD_REMAP | LBM_LAUNCH_WAVES4 | LBM_LAUNCH_WAVES3;
+
+// Unused LDS reserved per workgroup for LBM_LAUNCH_WAVES4 / WAVES3: caps the
+// resident workgroups of a CU's 160 KB at 4 / 3 waves per SIMD.
+size_t waves_lds(int opts, int block)
+{
+    const int waves = (opts & LBM_LAUNCH_WAVES3) ? 3 : (opts & LBM_LAUNCH_WAVES4) ? 4 : 0;
+    if (!waves)
+        return 0;
+    const int per_cu = std::max(1, waves * 4 / (block / 64));   // workgroups per CU
+    return (size_t)(160 * 1024) / (size_t)(per_cu + 1) + 1024;
+}
+
 bool valid_tuning(const lbm_batch_tuning& t)
 {
     return t.pairs >= 0 && t.pairs <= kPairShapes && (t.block == 256 || t.block == 512) &&
            (t.cells_per_thread == 1 || t.cells_per_thread == 2 || t.cells_per_thread == 4) &&
            t.waves_per_cu >= 0 && t.waves_per_cu <= 64 &&
-           !(t.options[0] & ~(LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_XCD_REMAP)) &&
-           !(t.options[1] & ~(LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_XCD_REMAP));
+           !(t.options[0] & ~kBatchOptions) && !(t.options[1] & ~kBatchOptions);
 }
 
 // Pair pass set-up: precision and boundary checks, second population
@@ -421,13 +434,16 @@ hipError_t launch_phase_t(const lbm_batch* b, int phase, hipStream_t st)
     const uint32_t grid = persistent ? std::min<uint32_t>(b->tiles, (uint32_t)t.waves_per_cu * b->cus)
                                      : b->tiles;
     const int xcd = (opts & LBM_LAUNCH_XCD_REMAP) ? 1 : 0;
+    const size_t lds = waves_lds(opts, BLOCK);
     // nontemporal accesses only for alpha (beta's shifted pattern loses with them)
     if (phase == 0 && (opts & LBM_LAUNCH_NONTEMPORAL))
-        hipLaunchKernelGGL((k_batch<P, BLOCK, true>), dim3(grid), dim3(BLOCK), 0, st, b->desc,
-                           b->tile_member, b->tiles, phase, t.cells_per_thread, persistent, xcd);
+        hipLaunchKernelGGL((k_batch<P, BLOCK, true>), dim3(grid), dim3(BLOCK), lds, st,
+                           b->desc, b->tile_member, b->tiles, phase, t.cells_per_thread,
+                           persistent, xcd);
     else
-        hipLaunchKernelGGL((k_batch<P, BLOCK, false>), dim3(grid), dim3(BLOCK), 0, st, b->desc,
-                           b->tile_member, b->tiles, phase, t.cells_per_thread, persistent, xcd);
+        hipLaunchKernelGGL((k_batch<P, BLOCK, false>), dim3(grid), dim3(BLOCK), lds, st,
+                           b->desc, b->tile_member, b->tiles, phase, t.cells_per_thread,
+                           persistent, xcd);
     return hipGetLastError();
 }
 
@@ -684,7 +700,7 @@ int lbm_batch_autotune(lbm_batch* b, int trial_steps, hipStream_t stream,
     for (int blk : blocks)
         for (int cpt : cpts)
             for (int wpc : wpcs)
-                for (int opt = 0; opt < 4; opt++) {
+                for (int opt : {0, 1, 2, 3, 4, 5, 8, 9}) {
                     cand.block = blk;
                     cand.cells_per_thread = cpt;
                     cand.waves_per_cu = wpc;

@@ -1,0 +1,66 @@
+"""Launch variants (nontemporal accesses, XCD renumbering, occupancy caps)
+change speed only: alpha and beta of a random state with every flag, gravity
+and a lid give the same bits under every option combination."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SIZE = (37, 21, 13)            # odd sizes: partial workgroups and wrap blocks
+
+
+def state(hip, precision, rng):
+    from cavityflow_cuda_amd import _lib
+    s = hip.LbmSolverHIP(SIZE, (_lib.OBSTACLE, _lib.GHOST_LAYER, _lib.OBSTACLE,
+                                _lib.VELOCITY_INJECTION, _lib.GHOST_LAYER, _lib.OBSTACLE),
+                         precision, inv_tau=1.37, lid_velocity=0.04,
+                         gravitation=(1e-4, -2e-4, 5e-5), store_densities=True,
+                         store_velocities=True)
+    nh = tuple(v + 2 for v in SIZE)
+    full = ((0, 0, 0), nh)
+    cells = int(np.prod(nh))
+    flags = rng.choice(np.array([1, 2, 4, 8], np.int32), cells, p=[0.15, 0.7, 0.1, 0.05])
+    s.setFlags(flags, *full)
+    w = np.array([1 / 18] * 6 + [1 / 36] * 12 + [1 / 3])
+    dd = (np.repeat(w, cells) * (1 + 0.05 * rng.standard_normal(19 * cells)))
+    s.setDensityDistributions(dd.astype(s.dtype), *full)
+    return s, full
+
+
+def run(s, full, alpha_opt, beta_opt):
+    from cavityflow_cuda_amd._lib import check, lib
+    check(lib().lbm_set_launch_options(s.handle, 0, alpha_opt))
+    check(lib().lbm_set_launch_options(s.handle, 1, beta_opt))
+    s.simulationStepAlpha()
+    a = s.getDensityDistributions(*full).copy()
+    s.simulationStepBeta()
+    b = s.getDensityDistributions(*full).copy()
+    return a, b, s.getVelocities().copy(), s.getDensities().copy()
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64", "f16s", "bf16s"])
+def test_every_launch_option_gives_the_same_bits(hip, precision):
+    print("[case] launch options", precision, flush=True)
+    want = None
+    for opt in range(16):
+        s, full = state(hip, precision, np.random.default_rng(11))
+        got = run(s, full, opt, (opt * 5) % 16)
+        s.close()
+        if want is None:
+            want = got
+            continue
+        for x, y in zip(got, want):
+            assert x.tobytes() == y.tobytes(), opt
+
+
+def test_default_options_are_valid_and_settable(hip):
+    from cavityflow_cuda_amd import LbmError
+    from cavityflow_cuda_amd._lib import check, lib
+    import ctypes
+    s = hip.LbmSolverHIP((8, 8, 8), precision="f32")
+    v = ctypes.c_int()
+    check(lib().lbm_get_launch_options(s.handle, 0, ctypes.byref(v)))
+    assert v.value == 1 | 8
+    with pytest.raises(LbmError):
+        check(lib().lbm_set_launch_options(s.handle, 0, 16))
+    s.close()
