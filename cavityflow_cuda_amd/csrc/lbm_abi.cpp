@@ -353,22 +353,8 @@ int lbm_create(const lbm_params* params, lbm_solver** out)
     s->stride = (n + 255) / 256 * 256;
     s->elem_bytes = eb;
     s->out_bytes = p.precision == LBM_F64 ? 8 : 4;
-    // measured on MI355X at 512^3 (per-kernel option sweep): FP32 gains from
-    // fewer resident waves, 16-bit storage from more; F64 beta from the XCD map
-    switch (p.precision) {
-    case LBM_F32:
-        s->launch[0] = LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_WAVES3;
-        s->launch[1] = LBM_LAUNCH_WAVES3;
-        break;
-    case LBM_F64:
-        s->launch[0] = LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_WAVES3;
-        s->launch[1] = LBM_LAUNCH_XCD_REMAP;
-        break;
-    default:
-        s->launch[0] = LBM_LAUNCH_NONTEMPORAL;
-        s->launch[1] = 0;
-        break;
-    }
+    s->launch[0] = default_launch(p.precision, 0);
+    s->launch[1] = default_launch(p.precision, 1);
     // round inv_tau / lid / gravity to the arithmetic type once
     if (p.precision != LBM_F64) {
         s->p.inv_tau = (float)p.inv_tau;
@@ -451,7 +437,7 @@ int lbm_set_launch_options(lbm_solver* s, int kernel, int options)
 {
     if (!s || kernel < 0 || kernel > 1 ||
         (options & ~(LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_XCD_REMAP | LBM_LAUNCH_WAVES4 |
-                     LBM_LAUNCH_WAVES3))) {
+                     LBM_LAUNCH_WAVES3 | LBM_LAUNCH_WAVES2))) {
         set_error("lbm_set_launch_options: invalid argument");
         return LBM_ERR_ARG;
     }

@@ -78,11 +78,12 @@ __device__ __forceinline__ void tile_range(uint32_t tiles, int persistent, int x
     }
 }
 
-template <class P, int BLOCK, bool NT>
+// One kernel per phase (PHASE 0 = alpha, 1 = beta), so that each compiles to
+// the registers of its own half-step only.
+template <class P, int BLOCK, bool NT, int PHASE>
 __global__ __launch_bounds__(BLOCK) void k_batch(const MemberDesc* __restrict__ mem,
                                                  const int32_t* __restrict__ tile_member,
-                                                 uint32_t tiles, int phase, int cpt,
-                                                 int persistent, int xcd)
+                                                 uint32_t tiles, int cpt, int persistent, int xcd)
 {
     using C = typename P::C;
     using S = typename P::S;
@@ -92,14 +93,14 @@ __global__ __launch_bounds__(BLOCK) void k_batch(const MemberDesc* __restrict__ 
     for (uint32_t t = first; t < tiles; t += step) {
         const int m = tile_member[t];
         const MemberDesc& d = mem[m];
-        const StepArgs<C> a = member_args<C>(d, phase);
+        const StepArgs<C> a = member_args<C>(d, PHASE);
         const uint32_t base = (t - d.tile_begin) * tile_cells;
-        if (phase == 0) {
+        if constexpr (PHASE == 0) {
             for (int k = 0; k < cpt; k++) {
                 const uint32_t id = base + (uint32_t)k * BLOCK + threadIdx.x;
                 if (id < (uint32_t)a.n) {
                     const Acc32<S, NT> acc{a.plane, id * (uint32_t)sizeof(S)};
-                    alpha_cell<P>(a, id, a.flags[id], acc);
+                    alpha_cell<P>(a, id, ((const LBM_GLOBAL uint8_t*)a.flags)[id], acc);
                 }
             }
         } else {
@@ -113,10 +114,10 @@ __global__ __launch_bounds__(BLOCK) void k_batch(const MemberDesc* __restrict__ 
                 if (wrap) {
                     // the wrap path addresses planes from the unshifted base
                     const Acc64<S, true> acc{(S*)d.plane[0][0], a.stride, (int32_t)id, a.n};
-                    beta_cell<P>(a, id, a.flags[id], acc);
+                    beta_cell<P>(a, id, ((const LBM_GLOBAL uint8_t*)a.flags)[id], acc);
                 } else {
                     const Acc32<S, false> acc{a.plane, id * (uint32_t)sizeof(S)};
-                    beta_cell<P>(a, id, a.flags[id], acc);
+                    beta_cell<P>(a, id, ((const LBM_GLOBAL uint8_t*)a.flags)[id], acc);
                 }
             }
         }
@@ -266,13 +267,16 @@ int upload_tiles(lbm_batch* b, uint32_t tile_cells, std::vector<MemberDesc>& des
 }
 
 constexpr int kBatchOptions =
-    LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_XCD_REMAP | LBM_LAUNCH_WAVES4 | LBM_LAUNCH_WAVES3;
+    LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_XCD_REMAP | LBM_LAUNCH_WAVES4 | LBM_LAUNCH_WAVES3 |
+    LBM_LAUNCH_WAVES2;
 
-// Unused LDS reserved per workgroup for LBM_LAUNCH_WAVES4 / WAVES3: caps the
-// resident workgroups of a CU's 160 KB at 4 / 3 waves per SIMD.
+// Unused LDS reserved per workgroup for LBM_LAUNCH_WAVES4 / WAVES3 / WAVES2:
+// caps the resident workgroups of a CU's 160 KB at 4 / 3 / 2 waves per SIMD.
 size_t waves_lds(int opts, int block)
 {
-    const int waves = (opts & LBM_LAUNCH_WAVES3) ? 3 : (opts & LBM_LAUNCH_WAVES4) ? 4 : 0;
+    const int waves = (opts & LBM_LAUNCH_WAVES2)   ? 2
+                      : (opts & LBM_LAUNCH_WAVES3) ? 3
+                      : (opts & LBM_LAUNCH_WAVES4) ? 4 : 0;
     if (!waves)
         return 0;
     const int per_cu = std::max(1, waves * 4 / (block / 64));   // workgroups per CU
@@ -436,14 +440,18 @@ hipError_t launch_phase_t(const lbm_batch* b, int phase, hipStream_t st)
     const int xcd = (opts & LBM_LAUNCH_XCD_REMAP) ? 1 : 0;
     const size_t lds = waves_lds(opts, BLOCK);
     // nontemporal accesses only for alpha (beta's shifted pattern loses with them)
-    if (phase == 0 && (opts & LBM_LAUNCH_NONTEMPORAL))
-        hipLaunchKernelGGL((k_batch<P, BLOCK, true>), dim3(grid), dim3(BLOCK), lds, st,
-                           b->desc, b->tile_member, b->tiles, phase, t.cells_per_thread,
-                           persistent, xcd);
+    if (phase != 0)
+        hipLaunchKernelGGL((k_batch<P, BLOCK, false, 1>), dim3(grid), dim3(BLOCK), lds, st,
+                           b->desc, b->tile_member, b->tiles, t.cells_per_thread, persistent,
+                           xcd);
+    else if (opts & LBM_LAUNCH_NONTEMPORAL)
+        hipLaunchKernelGGL((k_batch<P, BLOCK, true, 0>), dim3(grid), dim3(BLOCK), lds, st,
+                           b->desc, b->tile_member, b->tiles, t.cells_per_thread, persistent,
+                           xcd);
     else
-        hipLaunchKernelGGL((k_batch<P, BLOCK, false>), dim3(grid), dim3(BLOCK), lds, st,
-                           b->desc, b->tile_member, b->tiles, phase, t.cells_per_thread,
-                           persistent, xcd);
+        hipLaunchKernelGGL((k_batch<P, BLOCK, false, 0>), dim3(grid), dim3(BLOCK), lds, st,
+                           b->desc, b->tile_member, b->tiles, t.cells_per_thread, persistent,
+                           xcd);
     return hipGetLastError();
 }
 
@@ -557,8 +565,8 @@ int lbm_batch_create(lbm_solver* const* members, int count, lbm_batch** out)
     t.block = 256;
     t.cells_per_thread = 1;
     t.waves_per_cu = 0;
-    t.options[0] = LBM_LAUNCH_NONTEMPORAL;
-    t.options[1] = LBM_LAUNCH_XCD_REMAP;
+    t.options[0] = default_launch(b->precision, 0);
+    t.options[1] = default_launch(b->precision, 1);
     t.pairs = 0;
     if ((rc = apply_tuning(b, t))) {
         lbm_batch_destroy(b);
@@ -674,10 +682,13 @@ int lbm_batch_autotune(lbm_batch* b, int trial_steps, hipStream_t stream,
     int rc = set_dev(b->device);
     if (rc)
         return rc;
-    // Each phase of the in-place launches is tuned on its own; the pair pass
-    // is timed as a whole. The members' state advances by whole alpha + beta
-    // steps during the trials (an even number of pair passes), exactly as in
-    // a run. Times are per whole step, the first trial step left out.
+    // Trials time whole steps (alpha then beta), since each phase's speed
+    // depends on what the other leaves in the caches: first the workgroup
+    // shape with the current launch options, then alpha's options with beta's
+    // fixed, then beta's with the chosen alpha. The pair pass is timed as a
+    // whole. The members' state advances by whole steps during the trials
+    // (an even number of pair passes), exactly as in a run. Times are per
+    // whole step, the first trial step left out.
     hipEvent_t e0, e1;
     BATCH_HIP(hipEventCreate(&e0), "hipEventCreate");
     BATCH_HIP(hipEventCreate(&e1), "hipEventCreate");
@@ -689,47 +700,60 @@ int lbm_batch_autotune(lbm_batch* b, int trial_steps, hipStream_t stream,
         BATCH_HIP(hipEventElapsedTime(ms, e0, e1), "hipEventElapsedTime");
         return LBM_OK;
     };
-    const int blocks[] = {256, 512};
-    const int cpts[] = {1, 2, 4};
-    const int wpcs[] = {0, 4, 8, 16};
+    auto step_time = [&](const lbm_batch_tuning& c, float* per_step) -> int {
+        int r = apply_tuning(b, c);
+        if (r)
+            return r;
+        float total = 0.f;
+        for (int i = 0; i < trial_steps; i++) {
+            float t = 0.f;
+            auto both = [&] {
+                const hipError_t e = launch_phase(b, 0, stream);
+                return e != hipSuccess ? e : launch_phase(b, 1, stream);
+            };
+            if ((r = timed(both, &t)))
+                return r;
+            if (i > 0)
+                total += t;
+        }
+        *per_step = total / (float)(trial_steps - 1);
+        return LBM_OK;
+    };
     lbm_batch_tuning best = b->tuning;
     best.pairs = 0;
-    float best_ms[2] = {std::numeric_limits<float>::infinity(), std::numeric_limits<float>::infinity()};
+    float best_step = std::numeric_limits<float>::infinity();
+    auto consider = [&](const lbm_batch_tuning& c) -> int {
+        float t = 0.f;
+        const int r = step_time(c, &t);
+        if (r)
+            return r;
+        if (t < best_step) {
+            best_step = t;
+            best = c;
+        }
+        return LBM_OK;
+    };
+    const lbm_batch_tuning start = best;
+    for (int blk : {256, 512})
+        for (int cpt : {1, 2, 4})
+            for (int wpc : {0, 4, 8, 16}) {
+                lbm_batch_tuning cand = start;
+                cand.block = blk;
+                cand.cells_per_thread = cpt;
+                cand.waves_per_cu = wpc;
+                if ((rc = consider(cand)))
+                    return rc;
+            }
+    for (int ph = 0; ph < 2; ph++) {
+        const lbm_batch_tuning shape = best;
+        for (int opt : {0, 1, 2, 3, 4, 5, 8, 9, 16, 17}) {
+            lbm_batch_tuning cand = shape;
+            cand.options[ph] = opt;
+            if ((rc = consider(cand)))
+                return rc;
+        }
+    }
     lbm_batch_tuning cand;
-    cand.pairs = 0;
-    for (int blk : blocks)
-        for (int cpt : cpts)
-            for (int wpc : wpcs)
-                for (int opt : {0, 1, 2, 3, 4, 5, 8, 9}) {
-                    cand.block = blk;
-                    cand.cells_per_thread = cpt;
-                    cand.waves_per_cu = wpc;
-                    cand.options[0] = cand.options[1] = opt;
-                    if ((rc = apply_tuning(b, cand)))
-                        return rc;
-                    float ms[2] = {0.f, 0.f};
-                    for (int i = 0; i < trial_steps; i++)
-                        for (int ph = 0; ph < 2; ph++) {
-                            float t = 0.f;
-                            if ((rc = timed([&] { return launch_phase(b, ph, stream); }, &t)))
-                                return rc;
-                            if (i > 0)
-                                ms[ph] += t;
-                        }
-                    for (int ph = 0; ph < 2; ph++)
-                        if (ms[ph] < best_ms[ph]) {
-                            best_ms[ph] = ms[ph];
-                            if (ph == 0) {
-                                best.block = blk;
-                                best.cells_per_thread = cpt;
-                                best.waves_per_cu = wpc;
-                            }
-                            best.options[ph] = opt;
-                        }
-                }
-    // block / cells / grid shape are shared by both phases: keep alpha's, which
-    // carries more arithmetic; beta keeps its own launch options.
-    float best_step = best_ms[0] + best_ms[1];
     // the pair pass, when the members allow it (one pass = one whole step)
     bool eligible = true;
     for (int shape = 1; shape <= kPairShapes && eligible; shape++)
@@ -750,7 +774,7 @@ int lbm_batch_autotune(lbm_batch* b, int trial_steps, hipStream_t stream,
             if (i >= 2)
                 total += t;
         }
-        const float per_step = total / (float)(passes - 2) * (float)(trial_steps - 1);
+        const float per_step = total / (float)(passes - 2);
         if (per_step < best_step) {
             best_step = per_step;
             best = cand;

@@ -39,20 +39,26 @@ __device__ __forceinline__ uint32_t box_cell(const StepArgs<C>& a, uint32_t gid)
 // linear-index wrap (domainWrap, CLbmBetaCPU.cpp:44-50).
 // The plane bases come precomputed in the kernel arguments: opaque uniform
 // values the compiler cannot fold into per-lane pointer chains.
+// Every population array is device (global) memory. The accessors say so
+// explicitly: a base pointer read from memory (the batch member tables) is
+// otherwise a generic pointer, and generic accesses compile to flat
+// instructions, which wait on both memory counters.
+#define LBM_GLOBAL __attribute__((address_space(1)))
+
 template <class S, bool NT>
 struct Acc32 {
     const char* const* plane;
     uint32_t off;
     __device__ __forceinline__ S load(int l, int32_t) const
     {
-        const S* p = (const S*)(plane[l] + off);
+        const LBM_GLOBAL S* p = (const LBM_GLOBAL S*)(plane[l] + off);
         if (NT)
             return __builtin_nontemporal_load(p);
         return *p;
     }
     __device__ __forceinline__ void store(int l, int32_t, S v) const
     {
-        S* p = (S*)(const_cast<char*>(plane[l]) + off);
+        LBM_GLOBAL S* p = (LBM_GLOBAL S*)(const_cast<char*>(plane[l]) + off);
         if (NT)
             __builtin_nontemporal_store(v, p);
         else
@@ -74,8 +80,14 @@ struct Acc64 {
         }
         return (int64_t)l * stride + j;
     }
-    __device__ __forceinline__ S load(int l, int32_t delta) const { return dd[index(l, delta)]; }
-    __device__ __forceinline__ void store(int l, int32_t delta, S v) const { dd[index(l, delta)] = v; }
+    __device__ __forceinline__ S load(int l, int32_t delta) const
+    {
+        return ((const LBM_GLOBAL S*)dd)[index(l, delta)];
+    }
+    __device__ __forceinline__ void store(int l, int32_t delta, S v) const
+    {
+        ((LBM_GLOBAL S*)dd)[index(l, delta)] = v;
+    }
 };
 
 // ------------------------------------------------------- fused x packing ---
@@ -151,15 +163,17 @@ __device__ __forceinline__ void alpha_cell(const StepArgs<typename P::C>& a, uin
     uint32_t row;
     bool pl, pr;
     const bool pack = xpack_where(a, id, row, pl, pr);
-    // GHOST_LAYER is skipped (:75-76); OBSTACLE without outputs does nothing
-    if (!pack && (flag == kGhostLayer || (flag == kObstacle && !a.store_u && !a.store_rho)))
-        return;
-
+    // The slots are loaded for every cell, whatever its flag, so that the
+    // flag's latency overlaps theirs instead of preceding them. GHOST_LAYER
+    // (skipped by the reference, :75-76) and OBSTACLE without outputs then
+    // store nothing below.
     C d[kQ];
 #pragma unroll
     for (int l = 0; l < kQ; l++)
         d[l] = P::load(acc.load(l, 0), l);
-    if (flag == kGhostLayer) {   // reached only to pack its unchanged slots
+    // keeps the flag test (and its wait) below the issued loads
+    __builtin_amdgcn_sched_barrier(0);
+    if (flag == kGhostLayer) {   // packs its unchanged slots on a send plane
         xpack_alpha<P>(a, row, pl, pr, d);
         return;
     }
@@ -302,15 +316,15 @@ __device__ __forceinline__ void beta_cell(const StepArgs<typename P::C>& a, uint
     uint32_t row;
     bool pl, pr;
     const bool pack = xpack_where(a, id, row, pl, pr);
-    // OBSTACLE writes back what it reads; without outputs it does nothing
-    if (!pack && flag == kObstacle && !a.store_u && !a.store_rho)
-        return;
-
-    // plane l at id + c_l holds direction opp(l) arriving at this cell
+    // plane l at id + c_l holds direction opp(l) arriving at this cell. As in
+    // alpha_cell the slots are loaded whatever the flag (the caller's accessor
+    // keeps every neighbour index in bounds); OBSTACLE, which writes back what
+    // it reads, stores nothing below.
     C d[kQ];
 #pragma unroll
     for (int l = 0; l < kQ; l++)
         d[opp(l)] = P::load(acc.load(l, cx(l) + cy(l) * a.nx + cz(l) * a.nxy), l);
+    __builtin_amdgcn_sched_barrier(0);
 
     C rho, ux, uy, uz;
     moments(d, rho, ux, uy, uz);

@@ -216,19 +216,22 @@ static bool off32(const lbm_solver* s)
     return (uint64_t)s->stride * (uint64_t)s->elem_bytes < (1ull << 32);
 }
 
-// LBM_LAUNCH_WAVES4 / WAVES3: reserving this much (unused) LDS per 256-thread
-// workgroup leaves room for 4 (3) workgroups in a CU's 160 KB, i.e. 4 (3)
-// waves per SIMD. Fewer resident waves means fewer concurrent 19-plane
-// streams, which the in-place FP32 steps turn into higher DRAM efficiency.
+// LBM_LAUNCH_WAVES4 / WAVES3 / WAVES2: reserving this much (unused) LDS per
+// 256-thread workgroup leaves room for 4 (3, 2) workgroups in a CU's 160 KB,
+// i.e. 4 (3, 2) waves per SIMD. Fewer resident waves means fewer concurrent
+// 19-plane streams, which the in-place FP32 steps turn into higher DRAM
+// efficiency.
 constexpr size_t kWaves4Lds = 33 * 1024;
 constexpr size_t kWaves3Lds = 41 * 1024;
+constexpr size_t kWaves2Lds = 54 * 1024;
 
 template <class P, template <class, bool, bool, bool> class K, bool FULL>
 static void launch_variant(const lbm_solver* s, const StepArgs<typename P::C>& a, int opts,
                            hipStream_t st)
 {
     const uint32_t blocks = (a.cells + kBlock - 1) / kBlock;
-    const size_t lds = (opts & LBM_LAUNCH_WAVES3) ? kWaves3Lds
+    const size_t lds = (opts & LBM_LAUNCH_WAVES2)   ? kWaves2Lds
+                       : (opts & LBM_LAUNCH_WAVES3) ? kWaves3Lds
                        : (opts & LBM_LAUNCH_WAVES4) ? kWaves4Lds : 0;
     if (!off32(s))
         hipLaunchKernelGGL((K<P, FULL, false, false>::kernel), dim3(blocks), dim3(kBlock), lds,

@@ -162,9 +162,10 @@ int lbm_simulation_step_beta(lbm_solver* solver, const int origin[3], const int 
                              hipStream_t stream);
 
 /* Launch variants of the step kernels; they change speed, never results.
- * kernel: 0 = alpha, 1 = beta. Defaults (measured on MI355X at 512^3):
- * F32 alpha NONTEMPORAL | WAVES3, beta WAVES3; F64 alpha NONTEMPORAL |
- * WAVES3, beta XCD_REMAP; F16S / BF16S alpha NONTEMPORAL, beta none. No reference
+ * kernel: 0 = alpha, 1 = beta. Defaults (measured on MI355X at 512^3 with the
+ * six-face exchange): F32 alpha NONTEMPORAL | WAVES3, beta WAVES2; F64 alpha
+ * NONTEMPORAL | WAVES2, beta XCD_REMAP | WAVES2; F16S / BF16S alpha
+ * NONTEMPORAL, beta none. No reference
  * counterpart: the reference takes its block shapes from the XML
  * configuration (default.xml:93-115). */
 enum lbm_launch_option {
@@ -172,7 +173,8 @@ enum lbm_launch_option {
     LBM_LAUNCH_XCD_REMAP = 2,    /* one contiguous range of cells per XCD */
     LBM_LAUNCH_WAVES4 = 4,       /* at most 4 waves per SIMD (LDS reserved per
                                     workgroup): fewer concurrent plane streams */
-    LBM_LAUNCH_WAVES3 = 8        /* at most 3 waves per SIMD (wins over WAVES4) */
+    LBM_LAUNCH_WAVES3 = 8,       /* at most 3 waves per SIMD (wins over WAVES4) */
+    LBM_LAUNCH_WAVES2 = 16       /* at most 2 waves per SIMD (wins over both) */
 };
 int lbm_set_launch_options(lbm_solver* solver, int kernel, int options);
 int lbm_get_launch_options(const lbm_solver* solver, int kernel, int* options);

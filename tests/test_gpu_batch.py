@@ -29,6 +29,7 @@ TUNINGS = [dict(block=256, cells_per_thread=1, waves_per_cu=0, options=(1, 2)),
            dict(block=512, cells_per_thread=1, waves_per_cu=16, options=(2, 1)),
            dict(block=256, cells_per_thread=1, waves_per_cu=0, options=(9, 8)),
            dict(block=512, cells_per_thread=2, waves_per_cu=0, options=(4, 5)),
+           dict(block=256, cells_per_thread=2, waves_per_cu=0, options=(17, 16)),
            dict(block=256, cells_per_thread=1, waves_per_cu=0, options=(0, 0), pairs=1),
            dict(block=256, cells_per_thread=1, waves_per_cu=0, options=(2, 0), pairs=2),
            dict(block=256, cells_per_thread=1, waves_per_cu=0, options=(1, 0), pairs=3),

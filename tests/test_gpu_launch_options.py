@@ -42,9 +42,9 @@ def run(s, full, alpha_opt, beta_opt):
 def test_every_launch_option_gives_the_same_bits(hip, precision):
     print("[case] launch options", precision, flush=True)
     want = None
-    for opt in range(16):
+    for opt in range(32):
         s, full = state(hip, precision, np.random.default_rng(11))
-        got = run(s, full, opt, (opt * 5) % 16)
+        got = run(s, full, opt, (opt * 5) % 32)
         s.close()
         if want is None:
             want = got
@@ -53,14 +53,20 @@ def test_every_launch_option_gives_the_same_bits(hip, precision):
             assert x.tobytes() == y.tobytes(), opt
 
 
-def test_default_options_are_valid_and_settable(hip):
+# per-precision defaults (alpha, beta) documented in include/lbm_abi.h
+DEFAULTS = {"f32": (1 | 8, 16), "f64": (1 | 16, 2 | 16), "f16s": (1, 0), "bf16s": (1, 0)}
+
+
+@pytest.mark.parametrize("precision", sorted(DEFAULTS))
+def test_default_options_are_valid_and_settable(hip, precision):
     from cavityflow_cuda_amd import LbmError
     from cavityflow_cuda_amd._lib import check, lib
     import ctypes
-    s = hip.LbmSolverHIP((8, 8, 8), precision="f32")
+    s = hip.LbmSolverHIP((8, 8, 8), precision=precision)
     v = ctypes.c_int()
-    check(lib().lbm_get_launch_options(s.handle, 0, ctypes.byref(v)))
-    assert v.value == 1 | 8
+    for k in (0, 1):
+        check(lib().lbm_get_launch_options(s.handle, k, ctypes.byref(v)))
+        assert v.value == DEFAULTS[precision][k]
     with pytest.raises(LbmError):
-        check(lib().lbm_set_launch_options(s.handle, 0, 16))
+        check(lib().lbm_set_launch_options(s.handle, 0, 32))
     s.close()
