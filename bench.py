@@ -387,6 +387,9 @@ def main():
     ap.add_argument("--u-lid", type=float, default=0.1)
     ap.add_argument("--precision", default="f32", choices=sorted(BYTES_PER_POP))
     ap.add_argument("--no-halo", action="store_true")
+    ap.add_argument("--unpack", choices=("auto", "pair", "separate"), default="auto",
+                    help="x-face unpack: both faces in one call (pair) or one call each; "
+                         "auto = pair for 16-bit storage, separate otherwise")
     ap.add_argument("--no-fused-pack", action="store_true",
                     help="pack the x faces with separate kernels instead of inside the steps")
     ap.add_argument("--cpu-sample-steps", type=int, default=2)
@@ -471,14 +474,25 @@ def main():
         # the step kernels pack the LEFT/RIGHT faces themselves
         check(L.lbm_set_step_pack(solver.handle, bufs[0], bufs[1]))
 
+    # Measured at 512^3: one call for both x faces halves the cache lines the
+    # halo-plane unpack dirties, which pays for 16-bit storage; with FP32 the
+    # following beta absorbs the deferred write-back and loses what it saves.
+    separate_unpack = (args.unpack == "separate" or
+                       (args.unpack == "auto" and args.precision in ("f32", "f64")))
+
     def halo_exchange(phase):
         # reference order: x faces, then y, then z (CController.cpp:369-570)
         for axis in range(3):
             if not (fused and axis == 0):
                 for d in (2 * axis, 2 * axis + 1):
                     check(L.lbm_pack_face(solver.handle, d, phase, bufs[d], stream))
-            for d in (2 * axis, 2 * axis + 1):
-                check(L.lbm_unpack_face(shadow.handle, d ^ 1, phase, bufs[d], stream))
+            if separate_unpack:
+                for d in (2 * axis, 2 * axis + 1):
+                    check(L.lbm_unpack_face(shadow.handle, d ^ 1, phase, bufs[d], stream))
+            else:
+                # the low face receives what the high face sent, and vice versa
+                check(L.lbm_unpack_face_pair(shadow.handle, axis, phase, bufs[2 * axis + 1],
+                                             bufs[2 * axis], stream))
 
     step_fns = (L.lbm_simulation_step_alpha, L.lbm_simulation_step_beta)
     counter = [0]
@@ -567,7 +581,7 @@ def main():
             "tau": round(params["tau"], 6),
             "precision": args.precision,
             "halo_exchange": halo,
-            "fused_x_pack": fused,
+            "fused_x_pack": fused, "paired_unpack": halo and not separate_unpack,
             "parallelism": "single GPU (one rank's subdomain)",
             "launch_options": {"alpha": launch_opts[0], "beta": launch_opts[1]},
         },

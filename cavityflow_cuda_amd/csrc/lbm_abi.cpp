@@ -735,6 +735,34 @@ int lbm_unpack_face(lbm_solver* s, int direction, int phase, const void* dev_buf
     return LBM_OK;
 }
 
+int lbm_unpack_face_pair(lbm_solver* s, int axis, int phase, const void* lo_buf,
+                         const void* hi_buf, hipStream_t stream)
+{
+    if (!s || axis < 0 || axis > 2 || phase < 0 || phase > 1 || !lo_buf || !hi_buf) {
+        set_error("lbm_unpack_face_pair: invalid argument");
+        return LBM_ERR_ARG;
+    }
+    int rc;
+    if (axis != 0) {
+        if ((rc = lbm_unpack_face(s, 2 * axis, phase, lo_buf, stream)))
+            return rc;
+        return lbm_unpack_face(s, 2 * axis + 1, phase, hi_buf, stream);
+    }
+    if ((rc = set_device(s)))
+        return rc;
+    // the boxes of lbm_face_box for LEFT / RIGHT
+    const int nx = s->size_h[0];
+    const int x_lo = phase == 0 ? 0 : 1, x_hi = phase == 0 ? nx - 1 : nx - 2;
+    const uint32_t mask_lo = phase == 0 ? 0x7FFFFu : direction_mask(LBM_LEFT);
+    const uint32_t mask_hi = phase == 0 ? 0x7FFFFu : direction_mask(LBM_RIGHT);
+    // the halo planes (phase 0) share cache lines between the faces: cached
+    // stores merge them; the send planes (phase 1) share none: streaming stores
+    LBM_HIP(launch_set_dd_xpair(s, lo_buf, hi_buf, x_lo, x_hi, mask_lo, mask_hi, phase == 1,
+                                stream),
+            "pair unpack kernel");
+    return LBM_OK;
+}
+
 int lbm_set_step_pack(lbm_solver* s, void* left_buf, void* right_buf)
 {
     if (!s) {

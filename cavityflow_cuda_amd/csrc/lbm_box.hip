@@ -66,6 +66,41 @@ __global__ __launch_bounds__(kCopyBlock) void k_set_dd(typename P::S* __restrict
         dd[(int64_t)l * a.stride + id] = v;
 }
 
+// Both x faces of a solver in one launch: thread (l, r), r = 0..rows, writes
+// the high-face cell of row r - 1 and the low-face cell of row r. In the
+// halo planes (x = nx - 1 of one row, x = 0 of the next) the two cells are
+// adjacent, so a pair shares its cache line instead of each write dirtying
+// one line of its own. Planes outside a face's mask are left alone. NT:
+// streaming stores (for the send planes, where the two faces' planes differ).
+template <class P, bool NT>
+__global__ __launch_bounds__(kCopyBlock) void k_set_dd_xpair(typename P::S* __restrict__ dd,
+                                                             const typename P::C* __restrict__ lo,
+                                                             const typename P::C* __restrict__ hi,
+                                                             int64_t stride, int32_t nx,
+                                                             int32_t x_lo, int32_t x_hi,
+                                                             uint32_t rows, uint32_t mask_lo,
+                                                             uint32_t mask_hi)
+{
+    const int l = blockIdx.y;
+    const bool on_lo = (mask_lo >> l) & 1u, on_hi = (mask_hi >> l) & 1u;
+    if (!on_lo && !on_hi)
+        return;
+    const uint32_t r = blockIdx.x * kCopyBlock + threadIdx.x;
+    if (r > rows)
+        return;
+    typename P::S* p = dd + (int64_t)l * stride;
+    auto put = [&](int64_t i, typename P::S v) {
+        if (NT)
+            __builtin_nontemporal_store(v, p + i);
+        else
+            p[i] = v;
+    };
+    if (on_hi && r > 0)
+        put((int64_t)(r - 1) * nx + x_hi, P::store(hi[(int64_t)l * rows + r - 1], l));
+    if (on_lo && r < rows)
+        put((int64_t)r * nx + x_lo, P::store(lo[(int64_t)l * rows + r], l));
+}
+
 // rho / u fields (compute type, no conversion)
 template <class C>
 __global__ __launch_bounds__(kCopyBlock) void k_get_field(const C* __restrict__ f,
@@ -200,6 +235,43 @@ hipError_t launch_box_set(const lbm_solver* s, int field, const Box& b, const vo
     if (s->p.precision == LBM_F64)
         return field_t<double>(s, field, b, const_cast<void*>(src), false, st);
     return field_t<float>(s, field, b, const_cast<void*>(src), false, st);
+}
+
+template <class P>
+static hipError_t set_dd_xpair_t(const lbm_solver* s, const void* lo, const void* hi, int x_lo,
+                                 int x_hi, uint32_t mask_lo, uint32_t mask_hi, bool nt,
+                                 hipStream_t st)
+{
+    const uint32_t rows = (uint32_t)s->size_h[1] * (uint32_t)s->size_h[2];
+    dim3 grid((rows + 1 + kCopyBlock - 1) / kCopyBlock, kQ);
+    if (nt)
+        hipLaunchKernelGGL((k_set_dd_xpair<P, true>), grid, dim3(kCopyBlock), 0, st,
+                           (typename P::S*)s->dd, (const typename P::C*)lo,
+                           (const typename P::C*)hi, s->stride, s->size_h[0], x_lo, x_hi, rows,
+                           mask_lo, mask_hi);
+    else
+        hipLaunchKernelGGL((k_set_dd_xpair<P, false>), grid, dim3(kCopyBlock), 0, st,
+                           (typename P::S*)s->dd, (const typename P::C*)lo,
+                           (const typename P::C*)hi, s->stride, s->size_h[0], x_lo, x_hi, rows,
+                           mask_lo, mask_hi);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_dd_xpair(const lbm_solver* s, const void* lo, const void* hi, int x_lo,
+                               int x_hi, uint32_t mask_lo, uint32_t mask_hi, bool nt,
+                               hipStream_t st)
+{
+    switch (s->p.precision) {
+    case LBM_F32:
+        return set_dd_xpair_t<StoreF32>(s, lo, hi, x_lo, x_hi, mask_lo, mask_hi, nt, st);
+    case LBM_F64:
+        return set_dd_xpair_t<StoreF64>(s, lo, hi, x_lo, x_hi, mask_lo, mask_hi, nt, st);
+    case LBM_F16S:
+        return set_dd_xpair_t<StoreF16S>(s, lo, hi, x_lo, x_hi, mask_lo, mask_hi, nt, st);
+    case LBM_BF16S:
+        return set_dd_xpair_t<StoreBF16S>(s, lo, hi, x_lo, x_hi, mask_lo, mask_hi, nt, st);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_flags_get(const lbm_solver* s, const Box& b, int32_t* dst, hipStream_t st)

@@ -102,6 +102,10 @@ hipError_t launch_beta(const lbm_solver* s, const Box& b, hipStream_t st);
 // field: 0 populations (19 components, masked), 1 velocities (3), 2 densities (1)
 hipError_t launch_box_get(const lbm_solver* s, int field, const Box& b, void* dst,
                           hipStream_t st);
+// both x faces in one launch (nt: streaming stores)
+hipError_t launch_set_dd_xpair(const lbm_solver* s, const void* lo, const void* hi, int x_lo,
+                               int x_hi, uint32_t mask_lo, uint32_t mask_hi, bool nt,
+                               hipStream_t st);
 hipError_t launch_box_set(const lbm_solver* s, int field, const Box& b, const void* src,
                           uint32_t mask, hipStream_t st);
 hipError_t launch_flags_get(const lbm_solver* s, const Box& b, int32_t* dst, hipStream_t st);

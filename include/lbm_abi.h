@@ -255,6 +255,16 @@ int lbm_pack_face(lbm_solver* solver, int direction, int phase, void* dev_buf,
 int lbm_unpack_face(lbm_solver* solver, int direction, int phase, const void* dev_buf,
                     hipStream_t stream);
 
+/* Both faces of one axis (0 = x, 1 = y, 2 = z): the same result as
+ * lbm_unpack_face(2 * axis, phase, lo_buf) followed by
+ * lbm_unpack_face(2 * axis + 1, phase, hi_buf). For x, one launch writes the
+ * two faces row by row, so the halo cells at the end of one row and the
+ * start of the next (adjacent in memory) are written together. No reference
+ * counterpart: the reference unpacks each face with its own
+ * setDensityDistributions call (CController.cpp:411-435, 849-874). */
+int lbm_unpack_face_pair(lbm_solver* solver, int axis, int phase, const void* lo_buf,
+                         const void* hi_buf, hipStream_t stream);
+
 /* Fused halo packing of the two x faces into caller-owned device buffers of
  * lbm_face_elements(LEFT) elements each (NULL disables a side, both NULL
  * turns it off). While set, every whole-domain or box step also writes:

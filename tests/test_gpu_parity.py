@@ -241,6 +241,52 @@ def test_pack_unpack_faces_match_oracle(hip, oracle_cpu):
     s.close()
 
 
+@pytest.mark.parametrize("prec", ["f32", "f64", "f16s", "bf16s"])
+@pytest.mark.parametrize("size", [(9, 6, 7), (1, 4, 5)])
+def test_unpack_face_pair_equals_two_unpacks(hip, prec, size):
+    """lbm_unpack_face_pair(axis) leaves exactly the state of the two
+    single-face unpacks, for every axis and phase (x: the one-launch pair
+    kernel; the 3-wide x extent puts both phase-1 planes in one column)."""
+    from hiprt import DeviceBuffer
+    say("unpack_face_pair", prec, size)
+    rng = np.random.default_rng(31)
+    dt = np.float64 if prec == "f64" else np.float32
+    a = hip.LbmSolverHIP(size, (OBST,) * 6, prec)
+    b = hip.LbmSolverHIP(size, (OBST,) * 6, prec)
+    o, z = full_box(a)
+    cells = z[0] * z[1] * z[2]
+    w = np.array([1 / 18] * 4 + [1 / 36] * 12 + [1 / 18] * 2 + [1 / 3])
+    state = (w[:, None] * (1 + 0.1 * rng.standard_normal((19, cells)))).astype(dt)
+    for s in (a, b):
+        s.setDensityDistributions(state.ravel(), o, z)
+    for axis in range(3):
+        for phase in (0, 1):
+            n = a.face_elements(2 * axis)
+            src = [(w[:, None] * (1 + 0.1 * rng.standard_normal((19, n // 19)))).astype(dt)
+                   for _ in range(2)]
+            bufs = [DeviceBuffer(src[0].nbytes) for _ in range(2)]
+            for buf, v in zip(bufs, src):
+                buf.upload(v)
+            a.unpack_face(2 * axis, phase, bufs[0])
+            a.unpack_face(2 * axis + 1, phase, bufs[1])
+            b.unpack_face_pair(axis, phase, bufs[0], bufs[1])
+            a.sync()
+            b.sync()
+            for buf in bufs:
+                buf.free()
+            assert (a.getDensityDistributions(o, z).tobytes()
+                    == b.getDensityDistributions(o, z).tobytes()), (axis, phase)
+    from cavityflow_cuda_amd import LbmError
+    buf = DeviceBuffer(64)
+    with pytest.raises(LbmError):
+        b.unpack_face_pair(3, 0, buf, buf)
+    with pytest.raises(LbmError):
+        b.unpack_face_pair(0, 2, buf, buf)
+    buf.free()
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("prec,tol", [("f32", 2e-6), ("f64", 1e-13)])
 def test_mass_is_conserved_in_closed_box(hip, prec, tol):
     """No lid: the total of all populations (walls included) is invariant."""
