@@ -242,7 +242,7 @@ def batched_workload(args, cf, L, hip, stream):
         for sp in specs:
             sp.params = cf.cavity_parameters(sp.size, sp.u_lid, sp.reynolds, prec)
         b = CavityBatch(specs, prec)
-        tuning = b.autotune(trial_steps=3, stream=stream.value)
+        tuning = b.autotune(trial_steps=8, stream=stream.value)
         print(f"[batched] {prec} tuning {tuning}", file=sys.stderr, flush=True)
         b.run(100, stream=stream.value)      # warm-up: graph capture, then back to the start
         hip.sync(stream)
@@ -305,7 +305,7 @@ def serving_workload(args, cf, L, hip, stream):
     # capacity: the batch rate on a sample of the requests, with autotuned shapes
     sample = [CavitySpec(r.size, r.u_lid, r.reynolds, r.params) for r in reqs[:24]]
     b = CavityBatch(sample, "f32")
-    tuning = b.autotune(trial_steps=3, stream=stream.value)
+    tuning = b.autotune(trial_steps=8, stream=stream.value)
     b.run(100, stream=stream.value)
     hip.sync(stream)
     e0, e1 = hip.event(), hip.event()

@@ -90,11 +90,22 @@ __global__ __launch_bounds__(BLOCK) void k_batch(const MemberDesc* __restrict__ 
     uint32_t first, step;
     tile_range(tiles, persistent, xcd, first, step);
     const uint32_t tile_cells = (uint32_t)BLOCK * (uint32_t)cpt;
+    // A persistent workgroup strides over the tiles; consecutive tiles of a
+    // large member share its descriptor, which is then read only once.
+    int cur = -1;
+    StepArgs<C> a;
+    uint32_t tile_begin = 0;
+    const char* plane0 = nullptr;
     for (uint32_t t = first; t < tiles; t += step) {
         const int m = tile_member[t];
-        const MemberDesc& d = mem[m];
-        const StepArgs<C> a = member_args<C>(d, PHASE);
-        const uint32_t base = (t - d.tile_begin) * tile_cells;
+        if (m != cur) {
+            const MemberDesc& d = mem[m];
+            a = member_args<C>(d, PHASE);
+            tile_begin = d.tile_begin;
+            plane0 = d.plane[0][0];
+            cur = m;
+        }
+        const uint32_t base = (t - tile_begin) * tile_cells;
         if constexpr (PHASE == 0) {
             for (int k = 0; k < cpt; k++) {
                 const uint32_t id = base + (uint32_t)k * BLOCK + threadIdx.x;
@@ -113,7 +124,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch(const MemberDesc* __restrict__ 
                     continue;
                 if (wrap) {
                     // the wrap path addresses planes from the unshifted base
-                    const Acc64<S, true> acc{(S*)d.plane[0][0], a.stride, (int32_t)id, a.n};
+                    const Acc64<S, true> acc{(S*)plane0, a.stride, (int32_t)id, a.n};
                     beta_cell<P>(a, id, ((const LBM_GLOBAL uint8_t*)a.flags)[id], acc);
                 } else {
                     const Acc32<S, false> acc{a.plane, id * (uint32_t)sizeof(S)};
@@ -685,38 +696,30 @@ int lbm_batch_autotune(lbm_batch* b, int trial_steps, hipStream_t stream,
     // Trials time whole steps (alpha then beta), since each phase's speed
     // depends on what the other leaves in the caches: first the workgroup
     // shape with the current launch options, then alpha's options with beta's
-    // fixed, then beta's with the chosen alpha. The pair pass is timed as a
-    // whole. The members' state advances by whole steps during the trials
-    // (an even number of pair passes), exactly as in a run. Times are per
-    // whole step, the first trial step left out.
+    // fixed, then beta's with the chosen alpha, then the pair pass. The
+    // members' state advances by whole steps during the trials (an even
+    // number of pair passes), exactly as in a run.
     hipEvent_t e0, e1;
     BATCH_HIP(hipEventCreate(&e0), "hipEventCreate");
     BATCH_HIP(hipEventCreate(&e1), "hipEventCreate");
-    auto timed = [&](auto&& launch, float* ms) -> int {
-        BATCH_HIP(hipEventRecord(e0, stream), "hipEventRecord");
-        BATCH_HIP(launch(), "batch kernel launch");
-        BATCH_HIP(hipEventRecord(e1, stream), "hipEventRecord");
-        BATCH_HIP(hipEventSynchronize(e1), "hipEventSynchronize");
-        BATCH_HIP(hipEventElapsedTime(ms, e0, e1), "hipEventElapsedTime");
-        return LBM_OK;
-    };
-    auto step_time = [&](const lbm_batch_tuning& c, float* per_step) -> int {
+    // ms per whole step of candidate c, timed the way a run executes:
+    // lbm_batch_run of an even number of whole steps (graph-captured on a
+    // stream), once untimed (capture, warm-up) and once between two events
+    auto step_time = [&](const lbm_batch_tuning& c, int steps, float* per_step) -> int {
         int r = apply_tuning(b, c);
         if (r)
             return r;
-        float total = 0.f;
-        for (int i = 0; i < trial_steps; i++) {
-            float t = 0.f;
-            auto both = [&] {
-                const hipError_t e = launch_phase(b, 0, stream);
-                return e != hipSuccess ? e : launch_phase(b, 1, stream);
-            };
-            if ((r = timed(both, &t)))
-                return r;
-            if (i > 0)
-                total += t;
-        }
-        *per_step = total / (float)(trial_steps - 1);
+        const int whole = steps + (steps & 1);
+        if ((r = lbm_batch_run(b, 0, 2 * whole, stream)))
+            return r;
+        BATCH_HIP(hipEventRecord(e0, stream), "hipEventRecord");
+        if ((r = lbm_batch_run(b, 0, 2 * whole, stream)))
+            return r;
+        BATCH_HIP(hipEventRecord(e1, stream), "hipEventRecord");
+        BATCH_HIP(hipEventSynchronize(e1), "hipEventSynchronize");
+        float ms = 0.f;
+        BATCH_HIP(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        *per_step = ms / (float)whole;
         return LBM_OK;
     };
     lbm_batch_tuning best = b->tuning;
@@ -724,7 +727,7 @@ int lbm_batch_autotune(lbm_batch* b, int trial_steps, hipStream_t stream,
     float best_step = std::numeric_limits<float>::infinity();
     auto consider = [&](const lbm_batch_tuning& c) -> int {
         float t = 0.f;
-        const int r = step_time(c, &t);
+        const int r = step_time(c, trial_steps, &t);
         if (r)
             return r;
         if (t < best_step) {
@@ -736,7 +739,7 @@ int lbm_batch_autotune(lbm_batch* b, int trial_steps, hipStream_t stream,
     const lbm_batch_tuning start = best;
     for (int blk : {256, 512})
         for (int cpt : {1, 2, 4})
-            for (int wpc : {0, 4, 8, 16}) {
+            for (int wpc : {0, 1, 2, 3, 4, 8, 16}) {
                 lbm_batch_tuning cand = start;
                 cand.block = blk;
                 cand.cells_per_thread = cpt;
@@ -753,32 +756,44 @@ int lbm_batch_autotune(lbm_batch* b, int trial_steps, hipStream_t stream,
                 return rc;
         }
     }
-    lbm_batch_tuning cand;
-    // the pair pass, when the members allow it (one pass = one whole step)
+    // the pair pass, when the members allow it
     bool eligible = true;
     for (int shape = 1; shape <= kPairShapes && eligible; shape++)
-    for (int opt = 0; opt < 4; opt++) {
-        cand = best;
-        cand.pairs = shape;
-        cand.options[0] = opt;
-        if (apply_tuning(b, cand) != LBM_OK) {
-            eligible = false;   // not eligible: keep the in-place launches
-            break;
-        }
-        float total = 0.f;
-        const int passes = 2 * trial_steps;
-        for (int i = 0; i < passes; i++) {
+        for (int opt = 0; opt < 4; opt++) {
+            lbm_batch_tuning cand = best;
+            cand.pairs = shape;
+            cand.options[0] = opt;
+            if (apply_tuning(b, cand) != LBM_OK) {
+                eligible = false;   // not eligible: keep the in-place launches
+                break;
+            }
             float t = 0.f;
-            if ((rc = timed([&] { return launch_pair(b, i & 1, stream); }, &t)))
+            if ((rc = step_time(cand, trial_steps, &t)))
                 return rc;
-            if (i >= 2)
-                total += t;
+            if (t < best_step) {
+                best_step = t;
+                best = cand;
+            }
         }
-        const float per_step = total / (float)(passes - 2);
-        if (per_step < best_step) {
-            best_step = per_step;
-            best = cand;
+    // short trials are noisy: the winner must also beat the starting tuning
+    // over longer, interleaved runs
+    if (best.pairs != start.pairs || best.block != start.block ||
+        best.cells_per_thread != start.cells_per_thread ||
+        best.waves_per_cu != start.waves_per_cu || best.options[0] != start.options[0] ||
+        best.options[1] != start.options[1]) {
+        const int long_steps = 4 * trial_steps;
+        float t_best = 0.f, t_start = 0.f;
+        for (int rep = 0; rep < 2; rep++) {
+            float t = 0.f;
+            if ((rc = step_time(best, long_steps, &t)))
+                return rc;
+            t_best += t;
+            if ((rc = step_time(start, long_steps, &t)))
+                return rc;
+            t_start += t;
         }
+        if (t_start <= t_best)
+            best = start;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
