@@ -43,9 +43,12 @@ struct StoreF16S {
     using C = float;
     static constexpr bool kShifted = true;
     static constexpr float kScale = 4096.0f;
+    // (float)v * 2^-12 is exact, so the fused multiply-add rounds once, exactly
+    // as the separate multiply and add do; it compiles to one mixed-precision
+    // FMA that also converts v
     __device__ static __forceinline__ C load(S v, int l)
     {
-        return (float)v * (1.0f / kScale) + weight<float>(l);
+        return __builtin_fmaf((float)v, 1.0f / kScale, weight<float>(l));
     }
     __device__ static __forceinline__ S store(C v, int l)
     {
@@ -61,12 +64,11 @@ struct StoreBF16S {
     {
         return __uint_as_float(((uint32_t)b) << 16);
     }
-    // round to nearest even; inputs are finite
+    // round to nearest even (gfx950 converts in one instruction); inputs are
+    // finite
     __device__ static __forceinline__ uint16_t float_to_bf16(float f)
     {
-        uint32_t u = __float_as_uint(f);
-        u += 0x7FFFu + ((u >> 16) & 1u);
-        return (uint16_t)(u >> 16);
+        return __builtin_bit_cast(uint16_t, (__bf16)f);
     }
     __device__ static __forceinline__ C load(S v, int l) { return bf16_to_float(v) + weight<float>(l); }
     __device__ static __forceinline__ S store(C v, int l) { return float_to_bf16(v - weight<float>(l)); }

@@ -46,6 +46,45 @@ def test_shifted_16bit_storage_error_vs_fp64(hip, size, re, steps):
     assert errs["f32"] < errs["f16s"]
 
 
+W = np.array([1 / 18] * 4 + [1 / 36] * 12 + [1 / 18] * 2 + [1 / 3], np.float32)
+
+
+def _bf16_rne(x):
+    """float32 -> bfloat16 bits, round to nearest even (finite inputs)."""
+    u = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+@pytest.mark.parametrize("prec", ["f16s", "bf16s"])
+def test_shifted_codes_are_exact(hip, prec):
+    """The device's 16-bit codes and their decode, bit for bit against numpy:
+    F16S code = binary16 RNE of (f - w) * 2^12 (float32 arithmetic), decode =
+    float32(code) * 2^-12 + w; BF16S code = bfloat16 RNE of f - w, decode =
+    float32(code << 16) + w."""
+    print("[case] shifted codes", prec, flush=True)
+    s = hip.LbmSolverHIP((6, 5, 4), precision=prec)
+    full = ((0, 0, 0), s.size_h)
+    cells = int(np.prod(s.size_h))
+    rng = np.random.default_rng(5)
+    dev = rng.standard_normal((19, cells)).astype(np.float32) * np.float32(3e-3)
+    dev[:, ::7] *= np.float32(1e-3)                      # small deviations too
+    f = (W[:, None] + dev).astype(np.float32)
+    s.setDensityDistributions(f.ravel(), *full)
+    raw = s.save_state()[: 19 * cells * 2].view(np.uint16).reshape(19, cells)
+    d = (f - W[:, None]).astype(np.float32)
+    if prec == "f16s":
+        want = (d * np.float32(4096)).astype(np.float16).view(np.uint16)
+        back = (raw.view(np.float16).astype(np.float32) * np.float32(2.0 ** -12)
+                + W[:, None]).astype(np.float32)
+    else:
+        want = _bf16_rne(d)
+        back = ((raw.astype(np.uint32) << 16).view(np.float32) + W[:, None]).astype(np.float32)
+    assert np.array_equal(raw, want)
+    got = s.getDensityDistributions(*full).reshape(19, cells)
+    assert got.tobytes() == back.tobytes()
+    s.close()
+
+
 def test_shifted_storage_round_trip(hip):
     """Rest equilibrium stores exactly zero deviations; set/get of f values
     round to the 16-bit grid of f - w."""
