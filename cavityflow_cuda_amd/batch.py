@@ -107,9 +107,13 @@ class CavityBatch:
                     waves_per_cu=t.waves_per_cu, options=(t.options[0], t.options[1]),
                     pairs=t.pairs)
 
-    def set_tuning(self, block=256, cells_per_thread=1, waves_per_cu=0, options=(1, 2), pairs=0):
-        """Launch shape; pairs=1 runs each alpha + beta pair as one out-of-place
-        pass (then run() needs multiples of 4 steps)."""
+    def set_tuning(self, block=256, cells_per_thread=1, waves_per_cu=0, options=None, pairs=0):
+        """Launch shape; options = (alpha, beta) launch-option bits, None keeps
+        the current ones (initially the precision's defaults); pairs=1 runs
+        each alpha + beta pair as one out-of-place pass (then run() needs
+        multiples of 4 steps)."""
+        if options is None:
+            options = self.tuning()["options"]
         t = _lib.lbm_batch_tuning(block, cells_per_thread, waves_per_cu,
                                   (ctypes.c_int * 2)(*options), int(pairs))
         check(lib().lbm_batch_set_tuning(self._h, ctypes.byref(t)))

@@ -75,24 +75,32 @@ def test_graph_replay_equals_direct_launches(hip):
     stream_destroy(st)
 
 
-def test_autotune_then_reset_restores_initial_state(hip):
+@pytest.mark.parametrize("on_stream", [False, True])
+def test_autotune_then_reset_restores_initial_state(hip, on_stream):
+    """Autotune (its trials are graph-captured batch runs on a stream, direct
+    launches on the default stream), then reset: the batch continues exactly
+    like a fresh one."""
     from cavityflow_cuda_amd.batch import CavityBatch
-    print("[case] batch autotune", flush=True)
+    from hiprt import stream_create, stream_destroy
+    print("[case] batch autotune, stream" if on_stream else "[case] batch autotune", flush=True)
+    st = stream_create() if on_stream else None
     sp = specs(hip, "f32")
     fresh = CavityBatch(sp, "f32")
     b = CavityBatch(sp, "f32")
-    t = b.autotune(trial_steps=2)
+    t = b.autotune(trial_steps=2, stream=st)
     assert t["block"] in (256, 512) and t["cells_per_thread"] in (1, 2, 4)
     for i in range(len(sp)):
         assert b.moments(i).tobytes() == fresh.moments(i).tobytes()
-    b.run(40)
+    b.run(40, stream=st)
     fresh.run(40)
-    b.sync()
+    b.sync(st)
     fresh.sync()
     for i in range(len(sp)):
         assert b.moments(i).tobytes() == fresh.moments(i).tobytes()
     b.close()
     fresh.close()
+    if st is not None:
+        stream_destroy(st)
 
 
 def test_batch_residual_matches_solver_residual(hip):
