@@ -49,6 +49,9 @@ from .controller import cavity_parameters
 from .solver import LbmSolverHIP, _stream, device_bytes
 
 
+_ELEM_BYTES = {"f32": 4, "f64": 8, "f16s": 2, "bf16s": 2}
+
+
 @dataclass
 class SolveRequest:
     id: int
@@ -199,6 +202,12 @@ class CavityServer:
             return
         if req._bytes > self.budget:
             self._reject(req, f"needs {req._bytes} B, over the {self.budget} B budget")
+            return
+        # batch launches address a plane with 32-bit byte offsets
+        cells = int(np.prod([int(v) + 2 for v in req.size]))
+        plane = (cells + 255) // 256 * 256 * _ELEM_BYTES[self.precision]
+        if plane >= 1 << 32:
+            self._reject(req, f"a {plane} B population plane is too large for a batch member")
             return
         self.queue.append(req)
 

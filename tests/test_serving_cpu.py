@@ -47,6 +47,13 @@ def test_submit_rejections(hip):
     assert [srv.results[i].status for i in range(3)] == ["rejected"] * 3
     assert "multiple" in srv.results[0].message and "budget" in srv.results[2].message
     assert len(srv.queue) == 1 and srv.queue[0].id == 3 and 3 not in srv.results
+    # 1030^3 FP32 cells: a 4.37 GB plane, beyond the batch kernels' 32-bit offsets
+    big = CavityServer(hbm_budget=1e12)
+    big.submit(SolveRequest(4, (1028, 1028, 1028), 0.1, 3000.0))
+    big.submit(SolveRequest(5, (1000, 1000, 1000), 0.1, 3000.0))
+    assert big.results[4].status == "rejected"
+    assert "too large" in big.results[4].message
+    assert len(big.queue) == 1 and big.queue[0].id == 5
 
 
 def test_latency_summary():
