@@ -233,7 +233,7 @@ static void launch_variant(const lbm_solver* s, const StepArgs<typename P::C>& a
     const size_t lds = (opts & LBM_LAUNCH_WAVES2)   ? kWaves2Lds
                        : (opts & LBM_LAUNCH_WAVES3) ? kWaves3Lds
                        : (opts & LBM_LAUNCH_WAVES4) ? kWaves4Lds : 0;
-    if (!off32(s))
+    if (!off32(s) || (opts & LBM_LAUNCH_ADDR64))
         hipLaunchKernelGGL((K<P, FULL, false, false>::kernel), dim3(blocks), dim3(kBlock), lds,
                            st, a);
     else if (opts & LBM_LAUNCH_NONTEMPORAL)

@@ -174,7 +174,10 @@ enum lbm_launch_option {
     LBM_LAUNCH_WAVES4 = 4,       /* at most 4 waves per SIMD (LDS reserved per
                                     workgroup): fewer concurrent plane streams */
     LBM_LAUNCH_WAVES3 = 8,       /* at most 3 waves per SIMD (wins over WAVES4) */
-    LBM_LAUNCH_WAVES2 = 16       /* at most 2 waves per SIMD (wins over both) */
+    LBM_LAUNCH_WAVES2 = 16,      /* at most 2 waves per SIMD (wins over both) */
+    LBM_LAUNCH_ADDR64 = 32       /* 64-bit element indexing, as domains whose planes
+                                    exceed 4 GiB always use (single-domain kernels
+                                    only; results identical) */
 };
 int lbm_set_launch_options(lbm_solver* solver, int kernel, int options);
 int lbm_get_launch_options(const lbm_solver* solver, int kernel, int* options);

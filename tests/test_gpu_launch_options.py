@@ -1,6 +1,8 @@
-"""Launch variants (nontemporal accesses, XCD renumbering, occupancy caps)
-change speed only: alpha and beta of a random state with every flag, gravity
-and a lid give the same bits under every option combination."""
+"""Launch variants (nontemporal accesses, XCD renumbering, occupancy caps,
+64-bit indexing) change speed only: alpha and beta of a random state with
+every flag, gravity and a lid give the same bits under every option
+combination. The 64-bit indexing variant is the code path of domains whose
+planes exceed 4 GiB, checked here at a small size."""
 import numpy as np
 import pytest
 
@@ -42,9 +44,9 @@ def run(s, full, alpha_opt, beta_opt):
 def test_every_launch_option_gives_the_same_bits(hip, precision):
     print("[case] launch options", precision, flush=True)
     want = None
-    for opt in range(32):
+    for opt in range(64):
         s, full = state(hip, precision, np.random.default_rng(11))
-        got = run(s, full, opt, (opt * 5) % 32)
+        got = run(s, full, opt, (opt * 5) % 64)
         s.close()
         if want is None:
             want = got
@@ -68,5 +70,5 @@ def test_default_options_are_valid_and_settable(hip, precision):
         check(lib().lbm_get_launch_options(s.handle, k, ctypes.byref(v)))
         assert v.value == DEFAULTS[precision][k]
     with pytest.raises(LbmError):
-        check(lib().lbm_set_launch_options(s.handle, 0, 32))
+        check(lib().lbm_set_launch_options(s.handle, 0, 64))
     s.close()
