@@ -58,7 +58,13 @@ struct Acc32 {
     }
     __device__ __forceinline__ void store(int l, int32_t, S v) const
     {
-        LBM_GLOBAL S* p = (LBM_GLOBAL S*)(const_cast<char*>(plane[l]) + off);
+        // A fresh copy of the offset keeps the compiler from reusing the
+        // loads' 64-bit addresses (19 VGPR pairs kept live across the
+        // collision): each store then takes the uniform base + 32-bit offset
+        // form, as the loads do.
+        uint32_t o = off;
+        asm volatile("" : "+v"(o));
+        LBM_GLOBAL S* p = (LBM_GLOBAL S*)(const_cast<char*>(plane[l]) + o);
         if (NT)
             __builtin_nontemporal_store(v, p);
         else
