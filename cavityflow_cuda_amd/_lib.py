@@ -93,6 +93,7 @@ class lbm_batch_tuning(ctypes.Structure):
         ("waves_per_cu", ctypes.c_int),
         ("options", ctypes.c_int * 2),
         ("pairs", ctypes.c_int),
+        ("group_mb", ctypes.c_int),
     ]
 
 

@@ -105,17 +105,19 @@ class CavityBatch:
         check(lib().lbm_batch_get_tuning(self._h, ctypes.byref(t)))
         return dict(block=t.block, cells_per_thread=t.cells_per_thread,
                     waves_per_cu=t.waves_per_cu, options=(t.options[0], t.options[1]),
-                    pairs=t.pairs)
+                    pairs=t.pairs, group_mb=t.group_mb)
 
-    def set_tuning(self, block=256, cells_per_thread=1, waves_per_cu=0, options=None, pairs=0):
+    def set_tuning(self, block=256, cells_per_thread=1, waves_per_cu=0, options=None, pairs=0,
+                   group_mb=0):
         """Launch shape; options = (alpha, beta) launch-option bits, None keeps
         the current ones (initially the precision's defaults); pairs=1 runs
         each alpha + beta pair as one out-of-place pass (then run() needs
-        multiples of 4 steps)."""
+        multiples of 4 steps); group_mb > 0 runs the members in groups of at
+        most that many MiB of populations, one group after the other."""
         if options is None:
             options = self.tuning()["options"]
         t = _lib.lbm_batch_tuning(block, cells_per_thread, waves_per_cu,
-                                  (ctypes.c_int * 2)(*options), int(pairs))
+                                  (ctypes.c_int * 2)(*options), int(pairs), int(group_mb))
         check(lib().lbm_batch_set_tuning(self._h, ctypes.byref(t)))
 
     def autotune(self, trial_steps=3, stream=None) -> dict:

@@ -65,15 +65,16 @@ __device__ __forceinline__ StepArgs<C> member_args(const MemberDesc& d, int phas
     return a;
 }
 
-// Tile iteration: first tile, stride and count for this workgroup.
-__device__ __forceinline__ void tile_range(uint32_t tiles, int persistent, int xcd,
-                                           uint32_t& first, uint32_t& step)
+// Tile iteration over [tile_lo, tile_hi): first tile and stride of this
+// workgroup (one tile, or a persistent stride over the range).
+__device__ __forceinline__ void tile_range(uint32_t tile_lo, uint32_t tile_hi, int persistent,
+                                           int xcd, uint32_t& first, uint32_t& step)
 {
     if (!persistent) {
-        first = xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
-        step = tiles;   // one tile only
+        first = tile_lo + (xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x);
+        step = tile_hi;   // one tile only
     } else {
-        first = blockIdx.x;
+        first = tile_lo + blockIdx.x;
         step = gridDim.x;
     }
 }
@@ -83,12 +84,13 @@ __device__ __forceinline__ void tile_range(uint32_t tiles, int persistent, int x
 template <class P, int BLOCK, bool NT, int PHASE>
 __global__ __launch_bounds__(BLOCK) void k_batch(const MemberDesc* __restrict__ mem,
                                                  const int32_t* __restrict__ tile_member,
-                                                 uint32_t tiles, int cpt, int persistent, int xcd)
+                                                 uint32_t tile_lo, uint32_t tiles, int cpt,
+                                                 int persistent, int xcd)
 {
     using C = typename P::C;
     using S = typename P::S;
     uint32_t first, step;
-    tile_range(tiles, persistent, xcd, first, step);
+    tile_range(tile_lo, tiles, persistent, xcd, first, step);
     const uint32_t tile_cells = (uint32_t)BLOCK * (uint32_t)cpt;
     // A persistent workgroup strides over the tiles; consecutive tiles of a
     // large member share its descriptor, which is then read only once.
@@ -222,6 +224,8 @@ struct lbm_batch {
     MemberDesc* desc = nullptr;
     int32_t* tile_member = nullptr;
     uint32_t tiles = 0;        // tiles of the current tuning
+    // tile ranges [lo, hi) of the member groups (tuning.group_mb), in order
+    std::vector<std::pair<uint32_t, uint32_t>> groups;
     uint32_t res_tiles = 0;    // tiles of 1024 cells for the residual
     int32_t* res_member = nullptr;
     MemberDesc* res_desc = nullptr;
@@ -297,6 +301,7 @@ size_t waves_lds(int opts, int block)
 bool valid_tuning(const lbm_batch_tuning& t)
 {
     return t.pairs >= 0 && t.pairs <= kPairShapes && (t.block == 256 || t.block == 512) &&
+           t.group_mb >= 0 && !(t.pairs && t.group_mb) &&
            (t.cells_per_thread == 1 || t.cells_per_thread == 2 || t.cells_per_thread == 4) &&
            t.waves_per_cu >= 0 && t.waves_per_cu <= 64 &&
            !(t.options[0] & ~kBatchOptions) && !(t.options[1] & ~kBatchOptions);
@@ -400,8 +405,29 @@ int apply_tuning(lbm_batch* b, const lbm_batch_tuning& t)
             return rc;
     }
     b->tuning = t;
-    return upload_tiles(b, (uint32_t)(t.block * t.cells_per_thread), b->host_desc,
-                        &b->tile_member, &b->tiles, &b->desc);
+    const uint32_t tile_cells = (uint32_t)(t.block * t.cells_per_thread);
+    int rc = upload_tiles(b, tile_cells, b->host_desc, &b->tile_member, &b->tiles, &b->desc);
+    if (rc)
+        return rc;
+    // members in order, packed into groups of at most group_mb MiB of
+    // populations (a larger member alone); one group when group_mb is 0
+    b->groups.clear();
+    const uint64_t cap = (uint64_t)t.group_mb << 20;
+    uint64_t bytes = 0;
+    uint32_t lo = 0, hi = 0;
+    for (const lbm_solver* s : b->members) {
+        const uint64_t mb = (uint64_t)kQ * s->stride * s->elem_bytes;
+        const uint32_t mt = (uint32_t)((s->n + tile_cells - 1) / tile_cells);
+        if (cap && hi > lo && bytes + mb > cap) {
+            b->groups.push_back({lo, hi});
+            lo = hi;
+            bytes = 0;
+        }
+        hi += mt;
+        bytes += mb;
+    }
+    b->groups.push_back({lo, hi});
+    return LBM_OK;
 }
 
 template <class P, class G>
@@ -441,51 +467,69 @@ hipError_t launch_pair(const lbm_batch* b, int parity, hipStream_t st)
 }
 
 template <class P, int BLOCK>
-hipError_t launch_phase_t(const lbm_batch* b, int phase, hipStream_t st)
+hipError_t launch_tiles_t(const lbm_batch* b, uint32_t lo, uint32_t hi, int phase,
+                          hipStream_t st)
 {
     const lbm_batch_tuning& t = b->tuning;
     const int opts = t.options[phase];
     const int persistent = t.waves_per_cu > 0;
-    const uint32_t grid = persistent ? std::min<uint32_t>(b->tiles, (uint32_t)t.waves_per_cu * b->cus)
-                                     : b->tiles;
+    const uint32_t n = hi - lo;
+    const uint32_t grid = persistent ? std::min<uint32_t>(n, (uint32_t)t.waves_per_cu * b->cus) : n;
     const int xcd = (opts & LBM_LAUNCH_XCD_REMAP) ? 1 : 0;
     const size_t lds = waves_lds(opts, BLOCK);
     // nontemporal accesses only for alpha (beta's shifted pattern loses with them)
     if (phase != 0)
         hipLaunchKernelGGL((k_batch<P, BLOCK, false, 1>), dim3(grid), dim3(BLOCK), lds, st,
-                           b->desc, b->tile_member, b->tiles, t.cells_per_thread, persistent,
+                           b->desc, b->tile_member, lo, hi, t.cells_per_thread, persistent,
                            xcd);
     else if (opts & LBM_LAUNCH_NONTEMPORAL)
         hipLaunchKernelGGL((k_batch<P, BLOCK, true, 0>), dim3(grid), dim3(BLOCK), lds, st,
-                           b->desc, b->tile_member, b->tiles, t.cells_per_thread, persistent,
+                           b->desc, b->tile_member, lo, hi, t.cells_per_thread, persistent,
                            xcd);
     else
         hipLaunchKernelGGL((k_batch<P, BLOCK, false, 0>), dim3(grid), dim3(BLOCK), lds, st,
-                           b->desc, b->tile_member, b->tiles, t.cells_per_thread, persistent,
+                           b->desc, b->tile_member, lo, hi, t.cells_per_thread, persistent,
                            xcd);
     return hipGetLastError();
 }
 
 template <class P>
-hipError_t launch_phase_p(const lbm_batch* b, int phase, hipStream_t st)
+hipError_t launch_tiles_p(const lbm_batch* b, uint32_t lo, uint32_t hi, int phase,
+                          hipStream_t st)
 {
-    return b->tuning.block == 512 ? launch_phase_t<P, 512>(b, phase, st)
-                                  : launch_phase_t<P, 256>(b, phase, st);
+    return b->tuning.block == 512 ? launch_tiles_t<P, 512>(b, lo, hi, phase, st)
+                                  : launch_tiles_t<P, 256>(b, lo, hi, phase, st);
 }
 
-hipError_t launch_phase(const lbm_batch* b, int phase, hipStream_t st)
+// One half-step of the members whose tiles are [lo, hi).
+hipError_t launch_tiles(const lbm_batch* b, uint32_t lo, uint32_t hi, int phase, hipStream_t st)
 {
+    if (hi <= lo)
+        return hipSuccess;
     switch (b->precision) {
-    case LBM_F32: return launch_phase_p<StoreF32>(b, phase, st);
-    case LBM_F64: return launch_phase_p<StoreF64>(b, phase, st);
-    case LBM_F16S: return launch_phase_p<StoreF16S>(b, phase, st);
-    case LBM_BF16S: return launch_phase_p<StoreBF16S>(b, phase, st);
+    case LBM_F32: return launch_tiles_p<StoreF32>(b, lo, hi, phase, st);
+    case LBM_F64: return launch_tiles_p<StoreF64>(b, lo, hi, phase, st);
+    case LBM_F16S: return launch_tiles_p<StoreF16S>(b, lo, hi, phase, st);
+    case LBM_BF16S: return launch_tiles_p<StoreBF16S>(b, lo, hi, phase, st);
     default: return hipErrorInvalidValue;
     }
 }
 
+// One half-step of every member (one launch per group).
+hipError_t launch_phase(const lbm_batch* b, int phase, hipStream_t st)
+{
+    for (const auto& g : b->groups) {
+        const hipError_t e = launch_tiles(b, g.first, g.second, phase, st);
+        if (e != hipSuccess)
+            return e;
+    }
+    return hipSuccess;
+}
+
 // The launches of half_steps half-steps from first_phase: single half-steps,
 // or one pair pass per alpha + beta pair alternating between the buffers.
+// With several member groups, each group runs all its half-steps before the
+// next starts (members are independent, so any order gives the same bits).
 hipError_t enqueue(const lbm_batch* b, int first_phase, int half_steps, hipStream_t st)
 {
     hipError_t e = hipSuccess;
@@ -494,8 +538,9 @@ hipError_t enqueue(const lbm_batch* b, int first_phase, int half_steps, hipStrea
             e = launch_pair(b, i & 1, st);
         return e;
     }
-    for (int i = 0; i < half_steps && e == hipSuccess; i++)
-        e = launch_phase(b, (first_phase + i) & 1, st);
+    for (const auto& g : b->groups)
+        for (int i = 0; i < half_steps && e == hipSuccess; i++)
+            e = launch_tiles(b, g.first, g.second, (first_phase + i) & 1, st);
     return e;
 }
 
@@ -579,6 +624,7 @@ int lbm_batch_create(lbm_solver* const* members, int count, lbm_batch** out)
     t.options[0] = default_launch(b->precision, 0);
     t.options[1] = default_launch(b->precision, 1);
     t.pairs = 0;
+    t.group_mb = 0;
     if ((rc = apply_tuning(b, t))) {
         lbm_batch_destroy(b);
         return rc;
@@ -756,12 +802,24 @@ int lbm_batch_autotune(lbm_batch* b, int trial_steps, hipStream_t stream,
                 return rc;
         }
     }
+    // member groups small enough for the last-level cache to keep their state
+    // between consecutive steps
+    {
+        const lbm_batch_tuning ungrouped = best;
+        for (int mb : {32, 64, 128, 192, 256, 384}) {
+            lbm_batch_tuning cand = ungrouped;
+            cand.group_mb = mb;
+            if ((rc = consider(cand)))
+                return rc;
+        }
+    }
     // the pair pass, when the members allow it
     bool eligible = true;
     for (int shape = 1; shape <= kPairShapes && eligible; shape++)
         for (int opt = 0; opt < 4; opt++) {
             lbm_batch_tuning cand = best;
             cand.pairs = shape;
+            cand.group_mb = 0;
             cand.options[0] = opt;
             if (apply_tuning(b, cand) != LBM_OK) {
                 eligible = false;   // not eligible: keep the in-place launches
@@ -777,7 +835,8 @@ int lbm_batch_autotune(lbm_batch* b, int trial_steps, hipStream_t stream,
         }
     // short trials are noisy: the winner must also beat the starting tuning
     // over longer, interleaved runs
-    if (best.pairs != start.pairs || best.block != start.block ||
+    if (best.pairs != start.pairs || best.group_mb != start.group_mb ||
+        best.block != start.block ||
         best.cells_per_thread != start.cells_per_thread ||
         best.waves_per_cu != start.waves_per_cu || best.options[0] != start.options[0] ||
         best.options[1] != start.options[1]) {

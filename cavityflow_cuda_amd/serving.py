@@ -271,7 +271,8 @@ class CavityServer:
         if self.tuning:
             t = self.tuning
             bt = _lib.lbm_batch_tuning(t["block"], t["cells_per_thread"], t["waves_per_cu"],
-                                       (ctypes.c_int * 2)(*t["options"]), t.get("pairs", 0))
+                                       (ctypes.c_int * 2)(*t["options"]), t.get("pairs", 0),
+                                       t.get("group_mb", 0))
             check(lib().lbm_batch_set_tuning(self._batch, ctypes.byref(bt)))
         self._dirty = False
 

@@ -331,6 +331,15 @@ typedef struct lbm_batch_tuning {
                               all-OBSTACLE array boundary and no stored rho / u
                               outputs. options[0] applies to the pass
                               (NONTEMPORAL: streaming stores of the new state). */
+    int group_mb;          /* 0: every launch covers all members. > 0: members are
+                              packed, in order, into groups of at most group_mb MiB
+                              of populations (a larger member forms its own group);
+                              a half-step launches once per group, and a run
+                              advances one group through all its half-steps before
+                              the next, so a group's state can stay in the GPU's
+                              last-level cache between steps. Members are
+                              independent: the bits are the same in any schedule.
+                              Not with pairs. */
 } lbm_batch_tuning;
 
 int lbm_batch_create(lbm_solver* const* members, int count, lbm_batch** out);
@@ -346,9 +355,10 @@ int lbm_batch_step(lbm_batch* batch, int phase, hipStream_t stream);
  * first_phase must be 0 and half_steps a multiple of 4 (the state returns
  * to each member's own buffer). */
 int lbm_batch_run(lbm_batch* batch, int first_phase, int half_steps, hipStream_t stream);
-/* Times every candidate shape (block, cells_per_thread, waves_per_cu,
- * launch options, and the pair pass when the members allow it) for
- * trial_steps whole steps each and keeps the fastest.
+/* Times candidate tunings (block, cells_per_thread, waves_per_cu, launch
+ * options, member groups, and the pair pass when the members allow it) as
+ * runs of trial_steps whole steps (graph-captured on a stream) and keeps the
+ * fastest; the winner must also beat the starting tuning over longer runs.
  * The trials are real steps: the members advance; reset them afterwards
  * (lbm_reset) when their state matters. Synchronous. */
 int lbm_batch_autotune(lbm_batch* batch, int trial_steps, hipStream_t stream,

@@ -30,6 +30,8 @@ TUNINGS = [dict(block=256, cells_per_thread=1, waves_per_cu=0, options=(1, 2)),
            dict(block=256, cells_per_thread=1, waves_per_cu=0, options=(9, 8)),
            dict(block=512, cells_per_thread=2, waves_per_cu=0, options=(4, 5)),
            dict(block=256, cells_per_thread=2, waves_per_cu=0, options=(17, 16)),
+           dict(block=256, cells_per_thread=1, waves_per_cu=0, options=(9, 16), group_mb=1),
+           dict(block=512, cells_per_thread=2, waves_per_cu=2, options=(9, 16), group_mb=2),
            dict(block=256, cells_per_thread=1, waves_per_cu=0, options=(0, 0), pairs=1),
            dict(block=256, cells_per_thread=1, waves_per_cu=0, options=(2, 0), pairs=2),
            dict(block=256, cells_per_thread=1, waves_per_cu=0, options=(1, 0), pairs=3),
@@ -143,6 +145,11 @@ def test_pair_pass_rejections(hip):
     assert lib().lbm_batch_set_tuning(h, ctypes.byref(t)) == _lib.LBM_ERR_UNSUPPORTED
     t.pairs = 0
     check(lib().lbm_batch_set_tuning(h, ctypes.byref(t)))
+    # member groups and the pair pass do not combine; group_mb >= 0
+    t.pairs, t.group_mb = 1, 64
+    assert lib().lbm_batch_set_tuning(h, ctypes.byref(t)) == _lib.LBM_ERR_ARG
+    t.pairs, t.group_mb = 0, -1
+    assert lib().lbm_batch_set_tuning(h, ctypes.byref(t)) == _lib.LBM_ERR_ARG
     # the pair pass needs multiples of 4 half-steps from phase 0
     b2 = CavityBatch(specs(hip, "f32"), "f32")
     b2.set_tuning(pairs=1)
