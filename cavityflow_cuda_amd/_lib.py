@@ -32,7 +32,8 @@ EXPORTED = (
     "lbm_exchange_face", "lbm_device_alloc", "lbm_device_free",
     "lbm_set_launch_options", "lbm_get_launch_options", "lbm_set_step_pack", "lbm_get_moments",
     "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_size", "lbm_batch_set_tuning",
-    "lbm_batch_get_tuning", "lbm_batch_step", "lbm_batch_run", "lbm_batch_autotune",
+    "lbm_batch_get_tuning", "lbm_batch_step", "lbm_batch_run", "lbm_batch_launch",
+    "lbm_batch_autotune",
     "lbm_batch_residual", "lbm_reset", "lbm_device_bytes", "lbm_state_bytes",
     "lbm_save_state", "lbm_load_state",
     "lbm_reduce_box", "lbm_residual", "lbm_sync", "lbm_last_error", "lbm_status_string",
@@ -165,6 +166,7 @@ def lib() -> ctypes.CDLL:
         "lbm_batch_get_tuning": ([P, ctypes.POINTER(lbm_batch_tuning)], I),
         "lbm_batch_step": ([P, I, P], I),
         "lbm_batch_run": ([P, I, I, P], I),
+        "lbm_batch_launch": ([P, I, I, P], I),
         "lbm_batch_autotune": ([P, I, P, ctypes.POINTER(lbm_batch_tuning)], I),
         "lbm_batch_residual": ([P, I, ctypes.POINTER(ctypes.c_double), P], I),
         "lbm_reset": ([P], I),

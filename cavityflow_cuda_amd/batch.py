@@ -134,19 +134,21 @@ class CavityBatch:
         self.steps = 0
 
     # -- stepping -------------------------------------------------------------
-    def run(self, steps: int, stream=None, chunk: int = 200):
+    def run(self, steps: int, stream=None, chunk: int = 200, graph: bool = True):
         """`steps` time steps (alternating alpha and beta, as
         CController::computeNextStep counts them; even, so every member ends
         in the natural layout), replayed from a captured graph of `chunk`
-        steps on a non-default stream."""
+        steps on a non-default stream (graph=False: the same launches issued
+        directly)."""
         if steps % 2 or chunk % 2 or chunk <= 0:
             raise ValueError("steps and chunk must be even (whole alpha + beta pairs)")
         st = _stream(stream)
+        fn = lib().lbm_batch_run if graph else lib().lbm_batch_launch
         full, rest = divmod(int(steps), chunk)
         for _ in range(full):
-            check(lib().lbm_batch_run(self._h, 0, chunk, st))
+            check(fn(self._h, 0, chunk, st))
         if rest:
-            check(lib().lbm_batch_run(self._h, 0, rest, st))
+            check(fn(self._h, 0, rest, st))
         self.steps += int(steps)
 
     def sync(self, stream=None):

@@ -686,6 +686,24 @@ int lbm_batch_step(lbm_batch* b, int phase, hipStream_t stream)
     return LBM_OK;
 }
 
+int lbm_batch_launch(lbm_batch* b, int first_phase, int half_steps, hipStream_t stream)
+{
+    if (!b || first_phase < 0 || first_phase > 1 || half_steps < 0) {
+        set_error("lbm_batch_launch: invalid argument");
+        return LBM_ERR_ARG;
+    }
+    int rc = set_dev(b->device);
+    if (rc)
+        return rc;
+    if (b->tuning.pairs && (first_phase != 0 || half_steps % 4)) {
+        set_error("lbm_batch_launch: the pair pass needs first_phase 0 and a multiple of 4 "
+                  "half-steps");
+        return LBM_ERR_ARG;
+    }
+    BATCH_HIP(enqueue(b, first_phase, half_steps, stream), "batch kernel launch");
+    return LBM_OK;
+}
+
 int lbm_batch_run(lbm_batch* b, int first_phase, int half_steps, hipStream_t stream)
 {
     if (!b || first_phase < 0 || first_phase > 1 || half_steps < 0) {

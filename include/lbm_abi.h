@@ -355,6 +355,9 @@ int lbm_batch_step(lbm_batch* batch, int phase, hipStream_t stream);
  * first_phase must be 0 and half_steps a multiple of 4 (the state returns
  * to each member's own buffer). */
 int lbm_batch_run(lbm_batch* batch, int first_phase, int half_steps, hipStream_t stream);
+/* The launches of lbm_batch_run issued directly on the stream, without graph
+ * capture (same schedule, same bits). */
+int lbm_batch_launch(lbm_batch* batch, int first_phase, int half_steps, hipStream_t stream);
 /* Times candidate tunings (block, cells_per_thread, waves_per_cu, launch
  * options, member groups, and the pair pass when the members allow it) as
  * runs of trial_steps whole steps (graph-captured on a stream) and keeps the

@@ -57,23 +57,34 @@ def test_batch_members_equal_solo_runs(hip, precision, tuning):
     b.close()
 
 
-def test_graph_replay_equals_direct_launches(hip):
+@pytest.mark.parametrize("group_mb", [0, 1])
+def test_graph_replay_equals_direct_launches(hip, group_mb):
+    """Graph replays (several, with host work between) equal direct launches,
+    on the default stream and on a stream (lbm_batch_launch), with and without
+    member groups."""
     from hiprt import stream_create, stream_destroy
     from cavityflow_cuda_amd.batch import CavityBatch
-    print("[case] batch graph replay", flush=True)
+    print("[case] batch graph replay, group_mb", group_mb, flush=True)
     st = stream_create()
     sp = specs(hip, "f32")
     a = CavityBatch(sp, "f32")
     b = CavityBatch(sp, "f32")
+    c = CavityBatch(sp, "f32")
+    for x in (a, b, c):
+        x.set_tuning(group_mb=group_mb)
     for _ in range(4):                    # replay the same graph several times,
         a.run(20, stream=st, chunk=10)    # with host syncs and other work between
         a.sync(st)
         b.run(20, chunk=10)
         b.sync()
+        c.run(20, stream=st, chunk=10, graph=False)
+        c.sync(st)
         for i in range(len(sp)):
             assert a.moments(i).tobytes() == b.moments(i).tobytes()
+            assert c.moments(i).tobytes() == b.moments(i).tobytes()
     a.close()
     b.close()
+    c.close()
     stream_destroy(st)
 
 
