@@ -142,7 +142,6 @@ template <class P>
 __device__ __forceinline__ void xpack_beta(const StepArgs<typename P::C>& a, uint32_t row,
                                            bool pl, bool pr, const typename P::C (&v)[kQ])
 {
-    using C = typename P::C;
 #pragma unroll
     for (int l = 0; l < kQ - 1; l++) {
         const bool side = cx(l) < 0 ? pl : (cx(l) > 0 ? pr : false);
