@@ -763,9 +763,19 @@ int lbm_batch_autotune(lbm_batch* b, int trial_steps, hipStream_t stream,
     // fixed, then beta's with the chosen alpha, then the pair pass. The
     // members' state advances by whole steps during the trials (an even
     // number of pair passes), exactly as in a run.
-    hipEvent_t e0, e1;
-    BATCH_HIP(hipEventCreate(&e0), "hipEventCreate");
-    BATCH_HIP(hipEventCreate(&e1), "hipEventCreate");
+    // the two timing events live until every return path of the tuner
+    struct Events {
+        hipEvent_t e[2] = {nullptr, nullptr};
+        ~Events()
+        {
+            for (hipEvent_t x : e)
+                if (x)
+                    (void)hipEventDestroy(x);
+        }
+    } ev;
+    BATCH_HIP(hipEventCreate(&ev.e[0]), "hipEventCreate");
+    BATCH_HIP(hipEventCreate(&ev.e[1]), "hipEventCreate");
+    hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
     // ms per whole step of candidate c, timed the way a run executes:
     // lbm_batch_run of an even number of whole steps (graph-captured on a
     // stream), once untimed (capture, warm-up) and once between two events
@@ -872,8 +882,6 @@ int lbm_batch_autotune(lbm_batch* b, int trial_steps, hipStream_t stream,
         if (t_start <= t_best)
             best = start;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     if ((rc = apply_tuning(b, best)))
         return rc;
     if (chosen)
