@@ -30,7 +30,6 @@ struct MemberDesc {
     const uint8_t* flags;
     void* rho;
     void* u;
-    void* prev;                 // residual snapshot, 3 planes of the compute type
     int64_t stride;
     int32_t n, nx, nxy;
     uint32_t tile_begin;
@@ -139,72 +138,6 @@ __global__ __launch_bounds__(BLOCK) void k_batch(const MemberDesc* __restrict__ 
     }
 }
 
-// Per-member residual sums over FLUID cells: out[2m] += |u - u_prev|^2,
-// out[2m + 1] += |u|^2 (FP64, one atomic per workgroup and member tile).
-template <class P>
-__global__ __launch_bounds__(256) void k_batch_residual(const MemberDesc* __restrict__ mem,
-                                                        const int32_t* __restrict__ tile_member,
-                                                        uint32_t tiles, uint32_t tile_cells,
-                                                        int layout, double* out)
-{
-    using C = typename P::C;
-    using S = typename P::S;
-    __shared__ double lds[4][2];
-    for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
-        const int m = tile_member[t];
-        const MemberDesc& d = mem[m];
-        const S* dd = (const S*)d.plane[0][0];
-        C* prev = (C*)d.prev;
-        const uint32_t base = (t - d.tile_begin) * tile_cells;
-        double acc0 = 0.0, acc1 = 0.0;
-        for (uint32_t k = threadIdx.x; k < tile_cells; k += 256) {
-            const uint32_t id = base + k;
-            if (id >= (uint32_t)d.n || d.flags[id] != kFluid)
-                continue;
-            C f[kQ];
-#pragma unroll
-            for (int l = 0; l < kQ; l++) {
-                const C v = P::load(dd[(int64_t)l * d.stride + id], l);
-                if (layout)
-                    f[opp(l)] = v;
-                else
-                    f[l] = v;
-            }
-            C rho, ux, uy, uz;
-            moments(f, rho, ux, uy, uz);
-            const double dx = (double)ux - (double)prev[id];
-            const double dy = (double)uy - (double)prev[d.stride + id];
-            const double dz = (double)uz - (double)prev[2 * d.stride + id];
-            acc0 += dx * dx + dy * dy + dz * dz;
-            acc1 += (double)ux * ux + (double)uy * uy + (double)uz * uz;
-            prev[id] = ux;
-            prev[d.stride + id] = uy;
-            prev[2 * d.stride + id] = uz;
-        }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            acc0 += __shfl_xor(acc0, off, 64);
-            acc1 += __shfl_xor(acc1, off, 64);
-        }
-        if ((threadIdx.x & 63) == 0) {
-            lds[threadIdx.x >> 6][0] = acc0;
-            lds[threadIdx.x >> 6][1] = acc1;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            atomicAdd(&out[2 * m], lds[0][0] + lds[1][0] + lds[2][0] + lds[3][0]);
-            atomicAdd(&out[2 * m + 1], lds[0][1] + lds[1][1] + lds[2][1] + lds[3][1]);
-        }
-        __syncthreads();
-    }
-}
-
-__global__ void k_zero(double* p, int n)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n)
-        p[i] = 0.0;
-}
 
 }  // namespace lbm
 
@@ -226,10 +159,9 @@ struct lbm_batch {
     uint32_t tiles = 0;        // tiles of the current tuning
     // tile ranges [lo, hi) of the member groups (tuning.group_mb), in order
     std::vector<std::pair<uint32_t, uint32_t>> groups;
-    uint32_t res_tiles = 0;    // tiles of 1024 cells for the residual
-    int32_t* res_member = nullptr;
-    MemberDesc* res_desc = nullptr;
-    double* res_out = nullptr;
+    ResidualMember* res_mem = nullptr;   // members' residual inputs
+    double* res_part = nullptr;          // members * kReduceBlocks * 5 partial sums
+    double* res_out = nullptr;           // members * 5 sums
     PairDesc* pair_desc = nullptr;     // pair pass tables (tuning.pairs)
     int32_t* pair_member = nullptr;
     uint32_t pair_tiles = 0;
@@ -604,7 +536,6 @@ int lbm_batch_create(lbm_solver* const* members, int count, lbm_batch** out)
         d.flags = s->flags;
         d.rho = s->rho;
         d.u = s->u;
-        d.prev = nullptr;
         d.stride = s->stride;
         d.n = (int32_t)s->n;
         d.nx = nx;
@@ -640,7 +571,7 @@ int lbm_batch_destroy(lbm_batch* b)
     (void)set_dev(b->device);
     (void)hipDeviceSynchronize();
     drop_graphs(b);
-    void* bufs[] = {b->desc,    b->tile_member, b->res_member, b->res_desc,
+    void* bufs[] = {b->desc,    b->tile_member, b->res_mem,     b->res_part,
                     b->res_out, b->pair_desc,   b->pair_member, b->check_flag};
     for (void* p : bufs)
         if (p)
@@ -900,6 +831,7 @@ int lbm_batch_residual(lbm_batch* b, int layout, double* residuals, hipStream_t 
         return rc;
     const int count = (int)b->members.size();
     if (!b->res_out) {
+        std::vector<ResidualMember> mem(count);
         for (int i = 0; i < count; i++) {
             lbm_solver* s = b->members[i];
             if (!s->residual_prev) {
@@ -911,39 +843,23 @@ int lbm_batch_residual(lbm_batch* b, int layout, double* residuals, hipStream_t 
                 BATCH_HIP(hipMemset(s->residual_prev, 0, bytes), "hipMemset");
                 s->residual_valid = 0;
             }
-            b->host_desc[i].prev = s->residual_prev;
+            mem[i] = ResidualMember{s->dd, s->flags, s->residual_prev, s->stride,
+                                    (uint32_t)s->n};
         }
-        std::vector<MemberDesc> desc = b->host_desc;
-        if ((rc = upload_tiles(b, 1024, desc, &b->res_member, &b->res_tiles, &b->res_desc)))
-            return rc;
-        if (hipMalloc((void**)&b->res_out, (size_t)2 * count * sizeof(double)) != hipSuccess) {
+        if (hipMalloc((void**)&b->res_mem, mem.size() * sizeof(ResidualMember)) != hipSuccess ||
+            hipMalloc((void**)&b->res_part,
+                      (size_t)count * kReduceBlocks * 5 * sizeof(double)) != hipSuccess ||
+            hipMalloc((void**)&b->res_out, (size_t)count * 5 * sizeof(double)) != hipSuccess) {
             set_error("batch residual: out of device memory");
             return LBM_ERR_NOMEM;
         }
+        BATCH_HIP(hipMemcpy(b->res_mem, mem.data(), mem.size() * sizeof(ResidualMember),
+                            hipMemcpyHostToDevice), "H2D residual table");
     }
-    hipLaunchKernelGGL(k_zero, dim3((2 * count + 255) / 256), dim3(256), 0, stream, b->res_out,
-                       2 * count);
-    const uint32_t grid = std::min<uint32_t>(b->res_tiles, 8u * b->cus);
-    switch (b->precision) {
-    case LBM_F32:
-        hipLaunchKernelGGL(k_batch_residual<StoreF32>, dim3(grid), dim3(256), 0, stream,
-                           b->res_desc, b->res_member, b->res_tiles, 1024u, layout, b->res_out);
-        break;
-    case LBM_F64:
-        hipLaunchKernelGGL(k_batch_residual<StoreF64>, dim3(grid), dim3(256), 0, stream,
-                           b->res_desc, b->res_member, b->res_tiles, 1024u, layout, b->res_out);
-        break;
-    case LBM_F16S:
-        hipLaunchKernelGGL(k_batch_residual<StoreF16S>, dim3(grid), dim3(256), 0, stream,
-                           b->res_desc, b->res_member, b->res_tiles, 1024u, layout, b->res_out);
-        break;
-    default:
-        hipLaunchKernelGGL(k_batch_residual<StoreBF16S>, dim3(grid), dim3(256), 0, stream,
-                           b->res_desc, b->res_member, b->res_tiles, 1024u, layout, b->res_out);
-        break;
-    }
-    BATCH_HIP(hipGetLastError(), "batch residual launch");
-    std::vector<double> host(2 * count);
+    // every member's sums exactly as lbm_residual computes them for it alone
+    BATCH_HIP(launch_residual_members(b->precision, b->res_mem, count, layout, b->res_part,
+                                      b->res_out, stream), "batch residual launch");
+    std::vector<double> host(5 * (size_t)count);
     BATCH_HIP(hipMemcpyAsync(host.data(), b->res_out, host.size() * sizeof(double),
                              hipMemcpyDeviceToHost, stream), "D2H copy");
     BATCH_HIP(hipStreamSynchronize(stream), "hipStreamSynchronize");
@@ -955,8 +871,7 @@ int lbm_batch_residual(lbm_batch* b, int layout, double* residuals, hipStream_t 
             residuals[i] = std::numeric_limits<double>::infinity();
             s->residual_valid = 1;
         } else {
-            const double num = host[2 * i], den = host[2 * i + 1];
-            residuals[i] = residual_ratio(num, den);
+            residuals[i] = residual_ratio(host[5 * i], host[5 * i + 1]);
         }
     }
     return LBM_OK;

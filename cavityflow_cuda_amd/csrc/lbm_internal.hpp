@@ -118,6 +118,18 @@ hipError_t launch_moments(const lbm_solver* s, const Box& b, int layout, void* d
                           hipStream_t st);
 hipError_t launch_residual(const lbm_solver* s, int layout, double* partials, double* out_dev,
                            hipStream_t st);
+// One solver's residual inputs, for the residual of many solvers in one launch
+// (partials: count * kReduceBlocks * 5 doubles; out: count * 5).
+struct ResidualMember {
+    const void* dd;
+    const uint8_t* flags;
+    void* prev;
+    int64_t stride;
+    uint32_t n;
+};
+hipError_t launch_residual_members(int precision, const ResidualMember* mem, int count,
+                                   int layout, double* partials, double* out_dev,
+                                   hipStream_t st);
 
 constexpr int kReduceBlocks = 1024;
 

@@ -18,8 +18,9 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-// Block sum of kRedVals doubles into partials[blockIdx.x * kRedVals + k].
-__device__ __forceinline__ void block_store(double (&acc)[kRedVals], double* partials)
+// Block sum of kRedVals doubles into partials[block * kRedVals + k].
+__device__ __forceinline__ void block_store(double (&acc)[kRedVals], double* partials,
+                                            uint32_t block)
 {
     __shared__ double lds[kRedBlock / 64][kRedVals];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -34,7 +35,7 @@ __device__ __forceinline__ void block_store(double (&acc)[kRedVals], double* par
         double t = 0.0;
         for (int w = 0; w < kRedBlock / 64; w++)
             t += lds[w][threadIdx.x];
-        partials[blockIdx.x * kRedVals + threadIdx.x] = t;
+        partials[block * kRedVals + threadIdx.x] = t;
     }
 }
 
@@ -69,22 +70,23 @@ __global__ __launch_bounds__(kRedBlock) void k_reduce_box(const typename P::S* _
         }
         acc[4] += 1.0;
     }
-    block_store(acc, partials);
+    block_store(acc, partials, blockIdx.x);
 }
 
 // Residual over FLUID cells. layout 0: slot l holds direction l (after beta);
-// layout 1: slot l holds direction opp(l) (after alpha).
+// layout 1: slot l holds direction opp(l) (after alpha). Block `block` of
+// `nblocks` strides over the cells; its sums go to partials[block].
 template <class P>
-__global__ __launch_bounds__(kRedBlock) void k_residual(const typename P::S* __restrict__ dd,
-                                                        const uint8_t* __restrict__ flags,
-                                                        typename P::C* __restrict__ prev,
-                                                        int64_t stride, uint32_t n, int layout,
-                                                        double* partials)
+__device__ __forceinline__ void residual_block(const typename P::S* __restrict__ dd,
+                                               const uint8_t* __restrict__ flags,
+                                               typename P::C* __restrict__ prev,
+                                               int64_t stride, uint32_t n, int layout,
+                                               uint32_t block, uint32_t nblocks,
+                                               double* partials)
 {
     using C = typename P::C;
     double acc[kRedVals] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    for (uint32_t id = blockIdx.x * kRedBlock + threadIdx.x; id < n;
-         id += gridDim.x * kRedBlock) {
+    for (uint32_t id = block * kRedBlock + threadIdx.x; id < n; id += nblocks * kRedBlock) {
         if (flags[id] != kFluid)
             continue;
         C d[kQ];
@@ -108,7 +110,31 @@ __global__ __launch_bounds__(kRedBlock) void k_residual(const typename P::S* __r
         prev[stride + id] = uy;
         prev[2 * stride + id] = uz;
     }
-    block_store(acc, partials);
+    block_store(acc, partials, block);
+}
+
+template <class P>
+__global__ __launch_bounds__(kRedBlock) void k_residual(const typename P::S* __restrict__ dd,
+                                                        const uint8_t* __restrict__ flags,
+                                                        typename P::C* __restrict__ prev,
+                                                        int64_t stride, uint32_t n, int layout,
+                                                        double* partials)
+{
+    residual_block<P>(dd, flags, prev, stride, n, layout, blockIdx.x, gridDim.x, partials);
+}
+
+// The residual of many solvers in one launch: grid.y = member, and member m's
+// blocks do exactly what k_residual's do for that solver alone, into their own
+// slice of partials, so each member's sums carry the same bits as
+// lbm_residual's.
+template <class P>
+__global__ __launch_bounds__(kRedBlock) void k_residual_members(const ResidualMember* mem,
+                                                                int layout, double* partials)
+{
+    const ResidualMember& m = mem[blockIdx.y];
+    residual_block<P>((const typename P::S*)m.dd, m.flags, (typename P::C*)m.prev, m.stride,
+                      m.n, layout, blockIdx.x, gridDim.x,
+                      partials + (size_t)blockIdx.y * gridDim.x * kRedVals);
 }
 
 // Density and momentum of every cell of a box, computed from the populations
@@ -148,9 +174,12 @@ __global__ __launch_bounds__(kRedBlock) void k_moments(const typename P::S* __re
 // Sum of the per-block partials in a fixed order (lane i adds blocks i, i + 64,
 // ... in turn, then the butterfly over the wave): the same bits on every call,
 // in a few microseconds instead of one lane walking all blocks.
+// grid.x > 1: one slice of nblocks partials and one output per workgroup.
 __global__ __launch_bounds__(64) void k_finalize(const double* partials, int nblocks,
                                                  double* out)
 {
+    partials += (size_t)blockIdx.x * nblocks * kRedVals;
+    out += (size_t)blockIdx.x * kRedVals;
 #pragma unroll
     for (int k = 0; k < kRedVals; k++) {
         double t = 0.0;
@@ -234,6 +263,31 @@ hipError_t launch_reduce_box(const lbm_solver* s, const Box& b, double* partials
     case LBM_F64: return reduce_box_t<StoreF64>(s, b, partials, out_dev, st);
     case LBM_F16S: return reduce_box_t<StoreF16S>(s, b, partials, out_dev, st);
     case LBM_BF16S: return reduce_box_t<StoreBF16S>(s, b, partials, out_dev, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <class P>
+static hipError_t residual_members_t(const ResidualMember* mem, int count, int layout,
+                                     double* partials, double* out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_residual_members<P>, dim3(kReduceBlocks, count), dim3(kRedBlock), 0,
+                       st, mem, layout, partials);
+    hipLaunchKernelGGL(k_finalize, dim3(count), dim3(64), 0, st, partials, kReduceBlocks, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual_members(int precision, const ResidualMember* mem, int count,
+                                   int layout, double* partials, double* out_dev,
+                                   hipStream_t st)
+{
+    switch (precision) {
+    case LBM_F32: return residual_members_t<StoreF32>(mem, count, layout, partials, out_dev, st);
+    case LBM_F64: return residual_members_t<StoreF64>(mem, count, layout, partials, out_dev, st);
+    case LBM_F16S:
+        return residual_members_t<StoreF16S>(mem, count, layout, partials, out_dev, st);
+    case LBM_BF16S:
+        return residual_members_t<StoreBF16S>(mem, count, layout, partials, out_dev, st);
     default: return hipErrorInvalidValue;
     }
 }

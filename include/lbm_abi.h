@@ -366,7 +366,8 @@ int lbm_batch_launch(lbm_batch* batch, int first_phase, int half_steps, hipStrea
  * (lbm_reset) when their state matters. Synchronous. */
 int lbm_batch_autotune(lbm_batch* batch, int trial_steps, hipStream_t stream,
                        lbm_batch_tuning* chosen);
-/* lbm_residual for every member in one launch; residuals[count]. */
+/* lbm_residual for every member in one launch (each value bit-identical to
+ * the member's own lbm_residual); residuals[count]. */
 int lbm_batch_residual(lbm_batch* batch, int layout, double* residuals, hipStream_t stream);
 
 /* Populations back to the rest equilibrium, u = 0, rho = 1, flags untouched
