@@ -30,8 +30,10 @@ can never fit is rejected.
 arrival order, each on its own solver with one launch per half-step and the
 same residual test.
 
-Each member runs the same per-cell code as a lone solver, so a request's
-result is the same bits whether it ran alone or in a mix.
+Each member runs the same per-cell code as a lone solver, and the batch
+residual of a member carries the same bits as its own lbm_residual, so a
+request stops at the same step and returns the same bits whether it ran
+alone or in a mix.
 """
 from __future__ import annotations
 
