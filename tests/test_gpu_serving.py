@@ -49,7 +49,7 @@ def test_results_same_alone_and_in_the_mix(hip, precision, pairs):
         assert a.status in ("converged", "step_cap"), (i, a.message)
         assert a.fields.tobytes() == b.fields.tobytes(), i
         assert a.totals.tobytes() == b.totals.tobytes(), i
-        assert math.isclose(a.residual, b.residual, rel_tol=1e-9), i
+        assert a.residual == b.residual, i      # the batch residual has the same bits
     assert statuses == {"converged", "step_cap"}
     srv.close()
 
