@@ -1,6 +1,5 @@
 """Solve server on the GPU: results in a mix equal results alone (same bits),
 the HBM budget holds, and failure / cancel paths reclaim their slots."""
-import math
 
 import numpy as np
 import pytest
