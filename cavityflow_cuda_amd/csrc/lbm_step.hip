@@ -104,50 +104,67 @@ __global__ __launch_bounds__(kBlock) void k_init(StepArgs<typename P::C> a, int 
         a.rho[id] = rho;
 }
 
-template <class P, bool FULL, bool OFF32, bool NT>
-__global__ __launch_bounds__(kBlock) void k_alpha(StepArgs<typename P::C> a)
+// WIDE launches run 512-thread workgroups over 1024 consecutive cells, two
+// cells per thread one block apart; otherwise one cell per thread of a
+// 256-thread workgroup.
+template <bool WIDE>
+struct Shape {
+    static constexpr uint32_t kThreads = WIDE ? 2 * kBlock : kBlock;
+    static constexpr uint32_t kCells = WIDE ? 2 : 1;          // per thread
+    static constexpr uint32_t kSpan = kThreads * kCells;      // per workgroup
+};
+
+template <class P, bool FULL, bool OFF32, bool NT, bool WIDE>
+__global__ __launch_bounds__(Shape<WIDE>::kThreads) void k_alpha(StepArgs<typename P::C> a)
 {
     using S = typename P::S;
+    using G = Shape<WIDE>;
     const uint32_t blk = a.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
-    const uint32_t gid = blk * kBlock + threadIdx.x;
-    if (gid >= a.cells)
-        return;
-    const uint32_t id = FULL ? gid : box_cell(a, gid);
-    if (OFF32) {
-        const Acc32<S, NT> acc{a.plane, id * (uint32_t)sizeof(S)};
-        alpha_cell<P>(a, id, a.flags[id], acc);
-    } else {
-        const Acc64<S, false> acc{(S*)a.dd, a.stride, (int32_t)id, a.n};
-        alpha_cell<P>(a, id, a.flags[id], acc);
+    for (uint32_t k = 0; k < G::kCells; k++) {
+        const uint32_t gid = blk * G::kSpan + k * G::kThreads + threadIdx.x;
+        if (gid >= a.cells)
+            return;
+        const uint32_t id = FULL ? gid : box_cell(a, gid);
+        if (OFF32) {
+            const Acc32<S, NT> acc{a.plane, id * (uint32_t)sizeof(S)};
+            alpha_cell<P>(a, id, a.flags[id], acc);
+        } else {
+            const Acc64<S, false> acc{(S*)a.dd, a.stride, (int32_t)id, a.n};
+            alpha_cell<P>(a, id, a.flags[id], acc);
+        }
     }
 }
 
-template <class P, bool FULL, bool OFF32, bool NT>
-__global__ __launch_bounds__(kBlock) void k_beta(StepArgs<typename P::C> a)
+template <class P, bool FULL, bool OFF32, bool NT, bool WIDE>
+__global__ __launch_bounds__(Shape<WIDE>::kThreads) void k_beta(StepArgs<typename P::C> a)
 {
     using S = typename P::S;
+    using G = Shape<WIDE>;
     const uint32_t blk = a.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
-    const uint32_t gid = blk * kBlock + threadIdx.x;
     // One decision per workgroup: neighbours of every cell of this block stay
-    // inside [0, n) unless the block touches the first or last slab.
-    const uint32_t g_first = blk * kBlock;
-    const uint32_t g_last = min(g_first + kBlock - 1, a.cells - 1);
+    // inside [0, n) unless the block touches the first or last slab (box_cell
+    // is increasing in gid, so the first and last cell bound the others).
+    const uint32_t g_first = blk * G::kSpan;
+    const uint32_t g_last = min(g_first + G::kSpan - 1, a.cells - 1);
     const uint32_t id_first = FULL ? g_first : box_cell(a, g_first);
     const uint32_t id_last = FULL ? g_last : box_cell(a, g_last);
     const int64_t reach = (int64_t)a.nxy + a.nx + 1;
     const bool wrap = (int64_t)id_first < reach || (int64_t)id_last + reach >= (int64_t)a.n;
-    if (gid >= a.cells)
-        return;
-    const uint32_t id = FULL ? gid : box_cell(a, gid);
-    if (wrap) {
-        const Acc64<S, true> acc{(S*)a.dd, a.stride, (int32_t)id, a.n};
-        beta_cell<P>(a, id, a.flags[id], acc);
-    } else if (OFF32) {
-        const Acc32<S, NT> acc{a.plane, id * (uint32_t)sizeof(S)};
-        beta_cell<P>(a, id, a.flags[id], acc);
-    } else {
-        const Acc64<S, false> acc{(S*)a.dd, a.stride, (int32_t)id, a.n};
-        beta_cell<P>(a, id, a.flags[id], acc);
+    for (uint32_t k = 0; k < G::kCells; k++) {
+        const uint32_t gid = g_first + k * G::kThreads + threadIdx.x;
+        if (gid >= a.cells)
+            return;
+        const uint32_t id = FULL ? gid : box_cell(a, gid);
+        if (wrap) {
+            const Acc64<S, true> acc{(S*)a.dd, a.stride, (int32_t)id, a.n};
+            beta_cell<P>(a, id, a.flags[id], acc);
+        } else if (OFF32) {
+            const Acc32<S, NT> acc{a.plane, id * (uint32_t)sizeof(S)};
+            beta_cell<P>(a, id, a.flags[id], acc);
+        } else {
+            const Acc64<S, false> acc{(S*)a.dd, a.stride, (int32_t)id, a.n};
+            beta_cell<P>(a, id, a.flags[id], acc);
+        }
     }
 }
 
@@ -225,32 +242,49 @@ constexpr size_t kWaves4Lds = 33 * 1024;
 constexpr size_t kWaves3Lds = 41 * 1024;
 constexpr size_t kWaves2Lds = 54 * 1024;
 
-template <class P, template <class, bool, bool, bool> class K, bool FULL>
+// Reserved LDS for a waves-per-SIMD cap with WIDE (512-thread) workgroups:
+// 4 waves per SIMD leave room for 2 workgroups per CU, 3 or 2 for one.
+constexpr size_t kWideWaves4Lds = 54 * 1024;
+constexpr size_t kWideWaves2Lds = 81 * 1024;
+
+template <class P, template <class, bool, bool, bool, bool> class K, bool FULL, bool WIDE>
+static void launch_shape(const lbm_solver* s, const StepArgs<typename P::C>& a, int opts,
+                         hipStream_t st)
+{
+    using G = Shape<WIDE>;
+    const uint32_t blocks = (uint32_t)(((uint64_t)a.cells + G::kSpan - 1) / G::kSpan);
+    const bool cap2 = opts & (LBM_LAUNCH_WAVES2 | LBM_LAUNCH_WAVES3);
+    const size_t lds = WIDE ? (cap2 ? kWideWaves2Lds
+                               : (opts & LBM_LAUNCH_WAVES4) ? kWideWaves4Lds : 0)
+                            : (opts & LBM_LAUNCH_WAVES2)   ? kWaves2Lds
+                            : (opts & LBM_LAUNCH_WAVES3) ? kWaves3Lds
+                            : (opts & LBM_LAUNCH_WAVES4) ? kWaves4Lds : 0;
+    const dim3 grid(blocks), block(G::kThreads);
+    if (!off32(s) || (opts & LBM_LAUNCH_ADDR64))
+        hipLaunchKernelGGL((K<P, FULL, false, false, WIDE>::kernel), grid, block, lds, st, a);
+    else if (opts & LBM_LAUNCH_NONTEMPORAL)
+        hipLaunchKernelGGL((K<P, FULL, true, true, WIDE>::kernel), grid, block, lds, st, a);
+    else
+        hipLaunchKernelGGL((K<P, FULL, true, false, WIDE>::kernel), grid, block, lds, st, a);
+}
+
+template <class P, template <class, bool, bool, bool, bool> class K, bool FULL>
 static void launch_variant(const lbm_solver* s, const StepArgs<typename P::C>& a, int opts,
                            hipStream_t st)
 {
-    const uint32_t blocks = (a.cells + kBlock - 1) / kBlock;
-    const size_t lds = (opts & LBM_LAUNCH_WAVES2)   ? kWaves2Lds
-                       : (opts & LBM_LAUNCH_WAVES3) ? kWaves3Lds
-                       : (opts & LBM_LAUNCH_WAVES4) ? kWaves4Lds : 0;
-    if (!off32(s) || (opts & LBM_LAUNCH_ADDR64))
-        hipLaunchKernelGGL((K<P, FULL, false, false>::kernel), dim3(blocks), dim3(kBlock), lds,
-                           st, a);
-    else if (opts & LBM_LAUNCH_NONTEMPORAL)
-        hipLaunchKernelGGL((K<P, FULL, true, true>::kernel), dim3(blocks), dim3(kBlock), lds,
-                           st, a);
+    if (opts & LBM_LAUNCH_WIDE)
+        launch_shape<P, K, FULL, true>(s, a, opts, st);
     else
-        hipLaunchKernelGGL((K<P, FULL, true, false>::kernel), dim3(blocks), dim3(kBlock), lds,
-                           st, a);
+        launch_shape<P, K, FULL, false>(s, a, opts, st);
 }
 
-template <class P, bool FULL, bool OFF32, bool NT>
+template <class P, bool FULL, bool OFF32, bool NT, bool WIDE>
 struct AlphaK {
-    static constexpr auto kernel = k_alpha<P, FULL, OFF32, NT>;
+    static constexpr auto kernel = k_alpha<P, FULL, OFF32, NT, WIDE>;
 };
-template <class P, bool FULL, bool OFF32, bool NT>
+template <class P, bool FULL, bool OFF32, bool NT, bool WIDE>
 struct BetaK {
-    static constexpr auto kernel = k_beta<P, FULL, OFF32, NT>;
+    static constexpr auto kernel = k_beta<P, FULL, OFF32, NT, WIDE>;
 };
 
 template <class P>

@@ -163,7 +163,7 @@ int lbm_simulation_step_beta(lbm_solver* solver, const int origin[3], const int 
 
 /* Launch variants of the step kernels; they change speed, never results.
  * kernel: 0 = alpha, 1 = beta. Defaults (measured on MI355X at 512^3 with the
- * six-face exchange): F32 alpha NONTEMPORAL | WAVES3, beta WAVES2; F64 alpha
+ * six-face exchange): F32 alpha NONTEMPORAL | WAVES3, beta WIDE | WAVES2; F64 alpha
  * NONTEMPORAL | WAVES2, beta XCD_REMAP | WAVES2; F16S / BF16S alpha
  * NONTEMPORAL, beta none. No reference
  * counterpart: the reference takes its block shapes from the XML
@@ -175,9 +175,13 @@ enum lbm_launch_option {
                                     workgroup): fewer concurrent plane streams */
     LBM_LAUNCH_WAVES3 = 8,       /* at most 3 waves per SIMD (wins over WAVES4) */
     LBM_LAUNCH_WAVES2 = 16,      /* at most 2 waves per SIMD (wins over both) */
-    LBM_LAUNCH_ADDR64 = 32       /* 64-bit element indexing, as domains whose planes
+    LBM_LAUNCH_ADDR64 = 32,      /* 64-bit element indexing, as domains whose planes
                                     exceed 4 GiB always use (single-domain kernels
                                     only; results identical) */
+    LBM_LAUNCH_WIDE = 64         /* 512-thread workgroups, two cells per thread
+                                    (1024 consecutive cells per workgroup; with it
+                                    WAVES3 caps at 2 waves per SIMD, as WAVES2;
+                                    single-domain kernels only) */
 };
 int lbm_set_launch_options(lbm_solver* solver, int kernel, int options);
 int lbm_get_launch_options(const lbm_solver* solver, int kernel, int* options);

@@ -1,5 +1,5 @@
 """Launch variants (nontemporal accesses, XCD renumbering, occupancy caps,
-64-bit indexing) change speed only: alpha and beta of a random state with
+64-bit indexing, 512-thread workgroups of two cells per thread) change speed only: alpha and beta of a random state with
 every flag, gravity and a lid give the same bits under every option
 combination. The 64-bit indexing variant is the code path of domains whose
 planes exceed 4 GiB, checked here at a small size."""
@@ -44,9 +44,9 @@ def run(s, full, alpha_opt, beta_opt):
 def test_every_launch_option_gives_the_same_bits(hip, precision):
     print("[case] launch options", precision, flush=True)
     want = None
-    for opt in range(64):
+    for opt in range(128):
         s, full = state(hip, precision, np.random.default_rng(11))
-        got = run(s, full, opt, (opt * 5) % 64)
+        got = run(s, full, opt, (opt * 5) % 128)
         s.close()
         if want is None:
             want = got
@@ -56,7 +56,7 @@ def test_every_launch_option_gives_the_same_bits(hip, precision):
 
 
 # per-precision defaults (alpha, beta) documented in include/lbm_abi.h
-DEFAULTS = {"f32": (1 | 8, 16), "f64": (1 | 16, 2 | 16), "f16s": (1, 0), "bf16s": (1, 0)}
+DEFAULTS = {"f32": (1 | 8, 64 | 16), "f64": (1 | 16, 2 | 16), "f16s": (1, 0), "bf16s": (1, 0)}
 
 
 @pytest.mark.parametrize("precision", sorted(DEFAULTS))
@@ -70,5 +70,33 @@ def test_default_options_are_valid_and_settable(hip, precision):
         check(lib().lbm_get_launch_options(s.handle, k, ctypes.byref(v)))
         assert v.value == DEFAULTS[precision][k]
     with pytest.raises(LbmError):
-        check(lib().lbm_set_launch_options(s.handle, 0, 64))
+        check(lib().lbm_set_launch_options(s.handle, 0, 128))
     s.close()
+
+
+@pytest.mark.parametrize("precision", ["f32", "f16s"])
+def test_box_steps_same_bits_under_wide_workgroups(hip, precision):
+    """Sub-box steps (the exchange's inner and boundary boxes) map workgroup
+    cells through the box: 512-thread, two-cells-per-thread workgroups give
+    the bits of the 256-thread ones, partial last workgroup included."""
+    from cavityflow_cuda_amd._lib import LAUNCH_NONTEMPORAL, LAUNCH_WIDE, LAUNCH_WAVES2, check, lib
+    print("[case] wide box steps", precision, flush=True)
+    boxes = [((1, 1, 1), SIZE), ((2, 3, 1), (33, 15, 11)), ((1, 1, 1), (37, 21, 1)),
+             ((36, 2, 2), (2, 19, 11))]
+    want = None
+    for opts in (0, LAUNCH_WIDE, LAUNCH_WIDE | LAUNCH_NONTEMPORAL | LAUNCH_WAVES2):
+        s, full = state(hip, precision, np.random.default_rng(5))
+        check(lib().lbm_set_launch_options(s.handle, 0, opts))
+        check(lib().lbm_set_launch_options(s.handle, 1, opts & ~LAUNCH_NONTEMPORAL))
+        got = []
+        for o, z in boxes:
+            s.simulationStepAlpha(o, z)
+            got.append(s.getDensityDistributions(*full).copy())
+            s.simulationStepBeta(o, z)
+            got.append(s.getDensityDistributions(*full).copy())
+        s.close()
+        if want is None:
+            want = got
+            continue
+        for i, (x, y) in enumerate(zip(got, want)):
+            assert x.tobytes() == y.tobytes(), (opts, i)
