@@ -87,25 +87,31 @@ class CavityController:
             self.solvers[(i, j, k)] = s
 
     @classmethod
-    def from_config(cls, cfg, device=0, stream=None) -> "CavityController":
+    def from_config(cls, cfg, device=0, stream=None, vis_format="netcdf") -> "CavityController":
         """The cavity an XML configuration describes (``config.LbmConfiguration``):
         its size, subdomain split, CLbmSolver parameters and outputs, with a
-        VTK writer per rank when visualization is switched on (the reference
-        writes netCDF there, CController.cpp:120-135; netCDF is not part of
-        this build)."""
+        writer per rank when visualization is switched on: netCDF, the
+        reference's active writer (CController.cpp:120-135), or "vtk", the
+        writer it keeps commented out there."""
+        if vis_format not in ("netcdf", "vtk"):
+            raise ValueError(f"vis_format must be 'netcdf' or 'vtk', not {vis_format!r}")
         c = cls(cfg.domain_size, precision=cfg.precision, store_outputs=cfg.store_outputs,
                 split=cfg.subdomains, device=device, params=cfg.solver_parameters(),
                 stream=stream)
         c.config = cfg
         if cfg.do_visualization:
-            from .output import VtkWriter
+            from .output import NetCdfWriter, VtkWriter
             px, py, _ = cfg.subdomains
             for (i, j, k) in c.solvers:
                 rank = i + px * (j + py * k)
                 size, origin, length = cfg.subdomain(rank)
-                c.writers[(i, j, k)] = VtkWriter(rank, cfg.visualization_rate,
-                                                 cfg.visualization_dir, size, origin, length,
-                                                 cfg.precision)
+                if vis_format == "vtk":
+                    w = VtkWriter(rank, cfg.visualization_rate, cfg.visualization_dir, size,
+                                  origin, length, cfg.precision)
+                else:
+                    w = NetCdfWriter(rank, cfg.visualization_rate, cfg.visualization_dir, size,
+                                     length, cfg.precision)
+                c.writers[(i, j, k)] = w
         return c
 
     def render(self, iteration: int) -> list:

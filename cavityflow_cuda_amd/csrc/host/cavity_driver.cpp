@@ -5,15 +5,17 @@
 //   cavity_driver --size NX NY NZ [--split PX PY PZ] [--u-lid U] [--re RE]
 //                 [--loops N] [--precision f32|f64|f16s|bf16s] [--dump FILE]
 //   cavity_driver --config FILE.xml [--precision ...] [--loops N] [--dump FILE]
+//                 [--vis-format netcdf|vtk]
 //   cavity_driver --config FILE.xml --print-config
 //
 // * --config reads the reference's XML configuration (CConfiguration.cpp):
 //   size, subdomain split, physics, loops, outputs; the CLbmSolver parameters
 //   come from its physics for each rank's subdomain (CManager.cpp:28-58);
-//   with visualization on, every rank writes the VTK file of its interior
-//   at step 0 and after every step whose counter is a multiple of the rate
-//   (CController.cpp:1401-1430; the reference's active writer is netCDF,
-//   which this build does not have);
+//   with visualization on, every rank writes the file of its interior at
+//   step 0 and after every step whose counter is a multiple of the rate
+//   (CController.cpp:1401-1430): netCDF by default, the reference's active
+//   writer (CController.cpp:120-135), or VTK, the one it keeps commented out
+//   (--vis-format vtk);
 // * --print-config prints the parsed fields and the derived parameters of
 //   rank 0 and exits without touching the GPU;
 // * with validation on and more than one subdomain, the same configuration is
@@ -50,6 +52,7 @@
 
 #include "CLbmSolverHIP.hpp"
 #include "lbm_config.hpp"
+#include "lbm_netcdf.hpp"
 #include "lbm_vtk.hpp"
 
 namespace {
@@ -62,6 +65,7 @@ struct Options {
     std::string precision = "f32";
     std::string dump;
     std::string config;
+    std::string vis_format = "netcdf";
     bool print_config = false;
     bool loops_set = false;
 };
@@ -206,8 +210,12 @@ public:
             s->getVelocities(u.data());
             for (long c = 0; c < sc; c++)
                 flag_values[c] = flags[c];
-            lbm::write_vtk<T>(cfg_->visualization_dir, rank, (int)iteration, size, origin, len,
-                              flag_values.data(), rho.data(), u.data());
+            if (o_.vis_format == "vtk")
+                lbm::write_vtk<T>(cfg_->visualization_dir, rank, (int)iteration, size, origin,
+                                  len, flag_values.data(), rho.data(), u.data());
+            else
+                lbm::write_netcdf<T>(cfg_->visualization_dir, rank, (int)iteration, size, len,
+                                     flag_values.data(), rho.data(), u.data());
         }
     }
 
@@ -436,6 +444,13 @@ int main(int argc, char** argv)
         } else if (a == "--dump") {
             need(1);
             o.dump = argv[++i];
+        } else if (a == "--vis-format") {
+            need(1);
+            o.vis_format = argv[++i];
+            if (o.vis_format != "netcdf" && o.vis_format != "vtk") {
+                std::fprintf(stderr, "--vis-format takes netcdf or vtk\n");
+                return 2;
+            }
         } else {
             std::fprintf(stderr, "unknown argument %s\n", a.c_str());
             return 2;

@@ -7,10 +7,21 @@ arithmetic type, then CELL_DATA: flags (int), densities and velocities, in
 the reference's order and number formatting (C++ stream defaults: ``%g`` with
 6 significant digits). Fields come from the solver's stored outputs over the
 interior box, as the reference's getters deliver them.
+
+``visualization_<rank>_<iteration>.nc`` is the reference's active writer
+(CLbmVisualizationNetCDF.cpp:92-228, chosen in CController.cpp:120-135) in the
+netCDF classic format, written here without the netCDF library: dimensions
+x, y, z; coordinate variables x(x), y(y), z(z) at the cell centres
+(i + 0.5) * length / size in the arithmetic type; flags (int), densities and
+velocities_x / _y / _z over (z, y, x); a "units" attribute on each. Big-endian
+header, data right after it, variables in definition order, each padded to
+4 bytes; the 64-bit-offset variant (version byte 2) when an offset or a
+variable needs more than 31 bits.
 """
 from __future__ import annotations
 
 import os
+import struct
 
 import numpy as np
 
@@ -77,4 +88,94 @@ class VtkWriter:
         with open(p, "w") as f:
             f.write(vtk_text(self.rank, self.size, self.origin, self.length, flags, rho, u,
                              self.precision))
+        return p
+
+
+_NC_CHAR, _NC_INT, _NC_FLOAT, _NC_DOUBLE = 2, 4, 5, 6
+_NC_DIMENSION, _NC_VARIABLE, _NC_ATTRIBUTE = 10, 11, 12
+
+
+def _nc_text(s: str) -> bytes:
+    b = s.encode()
+    return struct.pack(">i", len(b)) + b + b"\0" * (-len(b) % 4)
+
+
+def netcdf_bytes(size, length, flags, rho, u, precision="f32", offset64=None) -> bytes:
+    """The .nc file contents for interior fields flags / rho [z][y][x] and
+    u [3][z][y][x] of a subdomain of `size` cells with physical `length`
+    (CLbmVisualizationNetCDF.cpp:92-155 layout, 166-228 values).
+    offset64: None chooses the format variant by size; True / False force it."""
+    T = np.dtype(np.float64 if precision == "f64" else np.float32)
+    tt = _NC_DOUBLE if T == np.float64 else _NC_FLOAT
+    nx, ny, nz = (int(v) for v in size)
+    cells = nx * ny * nz
+    coords = []
+    for a, n in enumerate((nx, ny, nz)):     # ((T)i + (T)0.5) * length / (T)size, in T
+        i = np.arange(n).astype(T)
+        coords.append((i + T.type(0.5)) * T.type(length[a]) / T.type(n))
+    uu = np.asarray(u, dtype=T).reshape(3, cells)
+    zyx = (2, 1, 0)
+    vars_ = [("x", (0,), tt, "meters", coords[0].astype(T.newbyteorder(">"))),
+             ("y", (1,), tt, "meters", coords[1].astype(T.newbyteorder(">"))),
+             ("z", (2,), tt, "meters", coords[2].astype(T.newbyteorder(">"))),
+             ("flags", zyx, _NC_INT, "flag", np.asarray(flags).reshape(cells).astype(">i4")),
+             ("densities", zyx, tt, "kg/m^3",
+              np.asarray(rho, dtype=T).reshape(cells).astype(T.newbyteorder(">"))),
+             ("velocities_x", zyx, tt, "m/s", uu[0].astype(T.newbyteorder(">"))),
+             ("velocities_y", zyx, tt, "m/s", uu[1].astype(T.newbyteorder(">"))),
+             ("velocities_z", zyx, tt, "m/s", uu[2].astype(T.newbyteorder(">")))]
+    vsize = [(v[4].nbytes + 3) & ~3 for v in vars_]
+    if max(vsize) > 2**32 - 4:
+        raise ValueError("netCDF: a variable of 4 GiB or more does not fit the classic format")
+    last_begin = sum(vsize[:-1])
+    if offset64 is None:
+        offset64 = last_begin + 1024 > 2**31 - 1 or max(vsize) > 2**31 - 4
+    head = [b"CDF" + bytes([2 if offset64 else 1]), struct.pack(">ii", 0, _NC_DIMENSION),
+            struct.pack(">i", 3)]
+    for name, n in (("x", nx), ("y", ny), ("z", nz)):
+        head += [_nc_text(name), struct.pack(">i", n)]
+    head += [struct.pack(">iii", 0, 0, _NC_VARIABLE), struct.pack(">i", len(vars_))]
+
+    def entries(begin):
+        out = []
+        for (name, dims, t, units, _), vs in zip(vars_, vsize):
+            out += [_nc_text(name), struct.pack(">i", len(dims)),
+                    struct.pack(f">{len(dims)}i", *dims),
+                    struct.pack(">ii", _NC_ATTRIBUTE, 1), _nc_text("units"),
+                    struct.pack(">i", _NC_CHAR), _nc_text(units), struct.pack(">ii", t, vs),
+                    struct.pack(">q" if offset64 else ">i", begin)]
+            begin += vs
+        return b"".join(out)
+
+    fixed = b"".join(head)
+    begin = len(fixed) + len(entries(0))
+    parts = [fixed, entries(begin)]
+    for (_, _, _, _, data), vs in zip(vars_, vsize):
+        parts += [data.tobytes(), b"\0" * (vs - data.nbytes)]
+    return b"".join(parts)
+
+
+class NetCdfWriter:
+    """CLbmVisualizationNetCDF<T> of the serial build: render(iteration)
+    writes visualization_<rank>_<iteration>.nc every `rate` iterations with
+    this rank's interior (the reference writes the fields at the subdomain's
+    global origin into a file sized for the subdomain, which only rank 0's
+    origin fits, and the coordinates from rank 0 only; here every rank's file
+    holds its own fields and coordinates)."""
+
+    def __init__(self, rank, rate, directory, size, length, precision="f32"):
+        self.rank, self.rate, self.directory = int(rank), int(rate), directory
+        self.size, self.length = tuple(size), tuple(length)
+        self.precision = precision
+
+    def path(self, iteration) -> str:
+        return os.path.join(self.directory, f"visualization_{self.rank}_{iteration}.nc")
+
+    def render(self, iteration, solver):
+        if iteration % self.rate:
+            return None
+        p = self.path(iteration)
+        with open(p, "wb") as f:
+            f.write(netcdf_bytes(self.size, self.length, solver.getFlags(),
+                                 solver.getDensities(), solver.getVelocities(), self.precision))
         return p

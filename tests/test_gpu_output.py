@@ -1,6 +1,7 @@
-"""A run from an XML configuration writes the same VTK files from the native
-C++ driver and from the Python controller (same kernels, same writer
-format), one file per rank at step 0 and every `rate` steps."""
+"""A run from an XML configuration writes the same visualization files (netCDF,
+the reference's active writer, and VTK) from the native C++ driver and from
+the Python controller (same kernels, same writer format), one file per rank
+at step 0 and every `rate` steps."""
 import os
 import subprocess
 
@@ -39,11 +40,11 @@ def test_driver_and_controller_write_identical_vtk(tmp_path, hip, precision):
     cfg_cpp.write_text(config_text(str(cpp_dir)))
     cfg_py = tmp_path / "py.xml"
     cfg_py.write_text(config_text(str(py_dir)))
-    r = subprocess.run([exe, "--config", str(cfg_cpp), "--precision", precision],
-                       capture_output=True, text=True, timeout=120)
+    r = subprocess.run([exe, "--config", str(cfg_cpp), "--precision", precision,
+                        "--vis-format", "vtk"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     cfg = LbmConfiguration.from_xml(str(cfg_py), precision)
-    c = hip.CavityController.from_config(cfg)
+    c = hip.CavityController.from_config(cfg, vis_format="vtk")
     c.run(cfg.loops)
     c.close()
     names = sorted(os.listdir(py_dir))
@@ -57,6 +58,56 @@ def test_driver_and_controller_write_identical_vtk(tmp_path, hip, precision):
     text = (py_dir / "visualization_1_6.vtk").read_text().splitlines()
     assert text[4] == "DIMENSIONS 7 11 9" and text[6].split()[0] == "0.6"
     assert "nan" not in (py_dir / "visualization_0_6.vtk").read_text()
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_driver_and_controller_write_identical_netcdf(tmp_path, hip, precision):
+    """netCDF is the default writer of both; each rank's file holds that
+    rank's interior fields as its solver returns them, read back with an
+    independent reader of the format."""
+    from scipy.io import netcdf_file
+    from cavityflow_cuda_amd import build
+    from cavityflow_cuda_amd.config import LbmConfiguration
+    print("[case] netcdf", precision, flush=True)
+    T = np.float64 if precision == "f64" else np.float32
+    exe = os.path.join(os.path.dirname(build.__file__), "cavity_driver")
+    cpp_dir, py_dir = tmp_path / "cpp", tmp_path / "py"
+    cpp_dir.mkdir()
+    py_dir.mkdir()
+    cfg_cpp = tmp_path / "cpp.xml"
+    cfg_cpp.write_text(config_text(str(cpp_dir)))
+    cfg_py = tmp_path / "py.xml"
+    cfg_py.write_text(config_text(str(py_dir)))
+    r = subprocess.run([exe, "--config", str(cfg_cpp), "--precision", precision],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    cfg = LbmConfiguration.from_xml(str(cfg_py), precision)
+    c = hip.CavityController.from_config(cfg)
+    c.run(cfg.loops)
+    c.sync()
+    fields = {}
+    for rank in (0, 1):
+        s = c.solvers[(rank, 0, 0)]
+        fields[rank] = (s.getFlags(), s.getDensities(), s.getVelocities())
+    c.close()
+    names = sorted(os.listdir(py_dir))
+    assert names == sorted(f"visualization_{r}_{i}.nc" for r in (0, 1) for i in (0, 3, 6))
+    assert sorted(os.listdir(cpp_dir)) == names
+    for n in names:
+        assert (cpp_dir / n).read_bytes() == (py_dir / n).read_bytes(), n
+    for rank in (0, 1):
+        f = netcdf_file(str(py_dir / f"visualization_{rank}_6.nc"), "r", mmap=False)
+        flags, rho, u = fields[rank]
+        assert f.dimensions == {"x": 6, "y": 10, "z": 8}
+        assert f.variables["flags"].data.ravel().tolist() == np.asarray(flags).ravel().tolist()
+        assert f.variables["densities"].data.astype(T).tobytes() == np.asarray(rho, T).tobytes()
+        for k, name in enumerate(("velocities_x", "velocities_y", "velocities_z")):
+            got = f.variables[name].data.astype(T).tobytes()
+            assert got == np.asarray(u, T).reshape(3, -1)[k].tobytes(), (rank, name)
+        # cells of length 0.1: centres 0.05, 0.15, ... from 0 in every rank's file
+        assert abs(float(f.variables["x"].data[0]) - 0.05) < 1e-6
+        assert abs(float(f.variables["x"].data[-1]) - 0.55) < 1e-6
+        f.close()
 
 
 @pytest.mark.parametrize("precision", ["f32", "f64"])
