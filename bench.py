@@ -541,6 +541,7 @@ def main():
     kern = {"alpha": float(alpha_ms.mean()), "beta": float(beta_ms.mean())}
     dom = max(kern, key=kern.get)
     achieved = alg_bytes / (kern[dom] * 1e-3) / 1e9
+    traffic = measured_traffic(f"k_{dom}", args.precision, args.size)
     step_ms_total = float(durs.sum())
     halo_ms = total_ms - step_ms_total
 
@@ -595,7 +596,10 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(f"k_{dom}", args.precision, args.size),
+            "traffic": traffic,
+            # measured HBM bytes (rocprof, per launch) over this run's launch time
+            "traffic_frac": (round(traffic / (kern[dom] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                             if traffic else None),
             "bytes_per_launch": alg_bytes,
             "mean_ms": {k: round(v, 4) for k, v in kern.items()},
         },
