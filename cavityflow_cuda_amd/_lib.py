@@ -18,6 +18,7 @@ F32, F64, F16S, BF16S = 0, 1, 2, 3
 LAUNCH_NONTEMPORAL, LAUNCH_XCD_REMAP, LAUNCH_WAVES4, LAUNCH_WAVES3, LAUNCH_WAVES2 = 1, 2, 4, 8, 16
 LAUNCH_ADDR64 = 32
 LAUNCH_WIDE = 64
+LAUNCH_LDS = 128
 LBM_OK = 0
 (LBM_ERR_ARG, LBM_ERR_BOX, LBM_ERR_HIP, LBM_ERR_NOMEM, LBM_ERR_PARAMS, LBM_ERR_STATE,
  LBM_ERR_UNSUPPORTED) = range(-1, -8, -1)

@@ -136,8 +136,8 @@ constexpr int kReduceBlocks = 1024;
 // Default launch options of the step kernels (kernel 0 = alpha, 1 = beta),
 // measured on MI355X at 512^3 with the six-face exchange: FP32 and FP64 gain
 // from fewer resident waves (fewer concurrent plane streams), 16-bit storage
-// needs every wave; F64 beta from the XCD map as well, FP32 beta from
-// 512-thread workgroups of two cells per thread.
+// needs every wave; FP32 beta from 512-thread workgroups of two cells per
+// thread, F64 beta from line-aligned accesses staged through LDS.
 inline int default_launch(int precision, int kernel)
 {
     switch (precision) {
@@ -146,7 +146,7 @@ inline int default_launch(int precision, int kernel)
                            : (LBM_LAUNCH_WIDE | LBM_LAUNCH_WAVES2);
     case LBM_F64:
         return kernel == 0 ? (LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_WAVES2)
-                           : (LBM_LAUNCH_XCD_REMAP | LBM_LAUNCH_WAVES2);
+                           : (LBM_LAUNCH_LDS | LBM_LAUNCH_WAVES2);
     default:
         return kernel == 0 ? LBM_LAUNCH_NONTEMPORAL : 0;
     }

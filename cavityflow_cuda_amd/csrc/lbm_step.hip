@@ -168,6 +168,85 @@ __global__ __launch_bounds__(Shape<WIDE>::kThreads) void k_beta(StepArgs<typenam
     }
 }
 
+// Beta with its plane accesses staged through LDS (LBM_LAUNCH_LDS). Plane l
+// of the cells [b, b + kBlock) lives at elements [b + s_l, b + s_l + kBlock)
+// with s_l = c_l . (1, nx, nx * ny): a range that starts mid cache line for
+// every moving direction. Each workgroup loads the 19 ranges from their
+// 128-byte line starts (every wave instruction then covers whole lines),
+// keeps them in LDS while every cell gathers, collides and scatters there
+// (a cell reads and writes only its own entries, as in place), and stores the
+// ranges back with the same aligned pattern, leaving out the elements of the
+// lines' other cells. Blocks near the ends of the array (the linear wrap) and
+// box steps use the per-cell path.
+template <class S>
+struct AccLds {
+    S (*tile)[kBlock];
+    uint32_t t;
+    __device__ __forceinline__ S load(int l, int32_t) const { return tile[l][t]; }
+    __device__ __forceinline__ void store(int l, int32_t, S v) const { tile[l][t] = v; }
+};
+
+template <class P, bool NT>
+__global__ __launch_bounds__(kBlock) void k_beta_lds(StepArgs<typename P::C> a)
+{
+    using S = typename P::S;
+    __shared__ S tile[kQ][kBlock];
+    const uint32_t blk = a.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const uint32_t b = blk * kBlock, t = threadIdx.x;
+    const uint32_t last = min(b + kBlock - 1, a.cells - 1);
+    const int64_t reach = (int64_t)a.nxy + a.nx + 1;
+    if ((int64_t)b < reach || (int64_t)last + reach >= (int64_t)a.n || last != b + kBlock - 1) {
+        // uniform per workgroup: no barrier on this path
+        const uint32_t id = b + t;
+        if (id < a.cells) {
+            const Acc64<S, true> acc{(S*)a.dd, a.stride, (int32_t)id, a.n};
+            beta_cell<P>(a, id, a.flags[id], acc);
+        }
+        return;
+    }
+    // the 19 ranges, each from the start of its first line: element j of the
+    // aligned run is tile position j - mis, valid for mis <= j < kBlock + mis
+    S head[kQ], tail[kQ];
+    uint32_t mis[kQ];
+#pragma unroll
+    for (int l = 0; l < kQ; l++) {
+        const char* base = a.plane[l] + (size_t)b * sizeof(S);
+        mis[l] = (uint32_t)(((uintptr_t)base & 127) / sizeof(S));
+        const LBM_GLOBAL S* run = (const LBM_GLOBAL S*)(base - mis[l] * sizeof(S));
+        head[l] = NT ? __builtin_nontemporal_load(run + t) : run[t];
+        if (t < mis[l])
+            tail[l] = NT ? __builtin_nontemporal_load(run + kBlock + t) : run[kBlock + t];
+    }
+#pragma unroll
+    for (int l = 0; l < kQ; l++) {
+        if (t >= mis[l])
+            tile[l][t - mis[l]] = head[l];
+        if (t < mis[l])
+            tile[l][kBlock - mis[l] + t] = tail[l];
+    }
+    __syncthreads();
+    const uint32_t id = b + t;
+    beta_cell<P>(a, id, a.flags[id], AccLds<S>{tile, t});
+    __syncthreads();
+#pragma unroll
+    for (int l = 0; l < kQ; l++) {
+        const char* base = a.plane[l] + (size_t)b * sizeof(S);
+        LBM_GLOBAL S* run = (LBM_GLOBAL S*)(const_cast<char*>(base) - mis[l] * sizeof(S));
+        if (t >= mis[l]) {
+            if (NT)
+                __builtin_nontemporal_store(tile[l][t - mis[l]], run + t);
+            else
+                run[t] = tile[l][t - mis[l]];
+        }
+        if (t < mis[l]) {
+            if (NT)
+                __builtin_nontemporal_store(tile[l][kBlock - mis[l] + t], run + kBlock + t);
+            else
+                run[kBlock + t] = tile[l][kBlock - mis[l] + t];
+        }
+    }
+}
+
 // -------------------------------------------------------------- launches ---
 
 template <class P>
@@ -299,6 +378,23 @@ static hipError_t alpha_t(const lbm_solver* s, const Box& b, hipStream_t st)
     return hipGetLastError();
 }
 
+// LBM_LAUNCH_LDS beta: the static tile counts towards the LDS reservation of
+// the waves-per-SIMD caps.
+template <class P>
+static void launch_beta_lds(const StepArgs<typename P::C>& a, int opts, hipStream_t st)
+{
+    constexpr size_t kTile = (size_t)kQ * kBlock * sizeof(typename P::S);
+    const size_t want = (opts & LBM_LAUNCH_WAVES2)   ? kWaves2Lds
+                        : (opts & LBM_LAUNCH_WAVES3) ? kWaves3Lds
+                        : (opts & LBM_LAUNCH_WAVES4) ? kWaves4Lds : 0;
+    const size_t lds = want > kTile ? want - kTile : 0;
+    const uint32_t blocks = (a.cells + kBlock - 1) / kBlock;
+    if (opts & LBM_LAUNCH_NONTEMPORAL)
+        hipLaunchKernelGGL((k_beta_lds<P, true>), dim3(blocks), dim3(kBlock), lds, st, a);
+    else
+        hipLaunchKernelGGL((k_beta_lds<P, false>), dim3(blocks), dim3(kBlock), lds, st, a);
+}
+
 template <class P>
 static hipError_t beta_t(const lbm_solver* s, const Box& b, hipStream_t st)
 {
@@ -308,6 +404,11 @@ static hipError_t beta_t(const lbm_solver* s, const Box& b, hipStream_t st)
     for (int l = 0; l < kQ; l++)
         a.plane[l] += (int64_t)(cx(l) + cy(l) * a.nx + cz(l) * a.nxy) *
                       (int64_t)sizeof(typename P::S);
+    const int opts = s->launch[1];
+    if ((opts & LBM_LAUNCH_LDS) && is_full(s, b) && off32(s) && !(opts & LBM_LAUNCH_ADDR64)) {
+        launch_beta_lds<P>(a, opts, st);
+        return hipGetLastError();
+    }
     if (is_full(s, b))
         launch_variant<P, BetaK, true>(s, a, s->launch[1], st);
     else

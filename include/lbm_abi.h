@@ -164,7 +164,7 @@ int lbm_simulation_step_beta(lbm_solver* solver, const int origin[3], const int 
 /* Launch variants of the step kernels; they change speed, never results.
  * kernel: 0 = alpha, 1 = beta. Defaults (measured on MI355X at 512^3 with the
  * six-face exchange): F32 alpha NONTEMPORAL | WAVES3, beta WIDE | WAVES2; F64 alpha
- * NONTEMPORAL | WAVES2, beta XCD_REMAP | WAVES2; F16S / BF16S alpha
+ * NONTEMPORAL | WAVES2, beta LDS | WAVES2; F16S / BF16S alpha
  * NONTEMPORAL, beta none. No reference
  * counterpart: the reference takes its block shapes from the XML
  * configuration (default.xml:93-115). */
@@ -178,10 +178,14 @@ enum lbm_launch_option {
     LBM_LAUNCH_ADDR64 = 32,      /* 64-bit element indexing, as domains whose planes
                                     exceed 4 GiB always use (single-domain kernels
                                     only; results identical) */
-    LBM_LAUNCH_WIDE = 64         /* 512-thread workgroups, two cells per thread
+    LBM_LAUNCH_WIDE = 64,        /* 512-thread workgroups, two cells per thread
                                     (1024 consecutive cells per workgroup; with it
                                     WAVES3 caps at 2 waves per SIMD, as WAVES2;
                                     single-domain kernels only) */
+    LBM_LAUNCH_LDS = 128         /* beta of the whole domain stages its shifted
+                                    plane ranges through LDS with line-aligned
+                                    accesses (wins over WIDE; alpha and box steps
+                                    ignore it) */
 };
 int lbm_set_launch_options(lbm_solver* solver, int kernel, int options);
 int lbm_get_launch_options(const lbm_solver* solver, int kernel, int* options);

@@ -1,5 +1,6 @@
 """Launch variants (nontemporal accesses, XCD renumbering, occupancy caps,
-64-bit indexing, 512-thread workgroups of two cells per thread) change speed only: alpha and beta of a random state with
+64-bit indexing, 512-thread workgroups of two cells per thread, LDS-staged
+beta) change speed only: alpha and beta of a random state with
 every flag, gravity and a lid give the same bits under every option
 combination. The 64-bit indexing variant is the code path of domains whose
 planes exceed 4 GiB, checked here at a small size."""
@@ -44,9 +45,9 @@ def run(s, full, alpha_opt, beta_opt):
 def test_every_launch_option_gives_the_same_bits(hip, precision):
     print("[case] launch options", precision, flush=True)
     want = None
-    for opt in range(128):
+    for opt in range(256):
         s, full = state(hip, precision, np.random.default_rng(11))
-        got = run(s, full, opt, (opt * 5) % 128)
+        got = run(s, full, opt, (opt * 5) % 256)
         s.close()
         if want is None:
             want = got
@@ -56,7 +57,7 @@ def test_every_launch_option_gives_the_same_bits(hip, precision):
 
 
 # per-precision defaults (alpha, beta) documented in include/lbm_abi.h
-DEFAULTS = {"f32": (1 | 8, 64 | 16), "f64": (1 | 16, 2 | 16), "f16s": (1, 0), "bf16s": (1, 0)}
+DEFAULTS = {"f32": (1 | 8, 64 | 16), "f64": (1 | 16, 128 | 16), "f16s": (1, 0), "bf16s": (1, 0)}
 
 
 @pytest.mark.parametrize("precision", sorted(DEFAULTS))
@@ -70,7 +71,7 @@ def test_default_options_are_valid_and_settable(hip, precision):
         check(lib().lbm_get_launch_options(s.handle, k, ctypes.byref(v)))
         assert v.value == DEFAULTS[precision][k]
     with pytest.raises(LbmError):
-        check(lib().lbm_set_launch_options(s.handle, 0, 128))
+        check(lib().lbm_set_launch_options(s.handle, 0, 256))
     s.close()
 
 

@@ -339,7 +339,7 @@ def test_seeded_random_cases_match_oracle(hip, oracle_cpu, dtype, seed):
     s = hip.LbmSolverHIP(size, (OBST,) * 6, PREC[dtype], kw["inv_tau"], kw["lid_velocity"],
                          kw["gravitation"])
     for k in (0, 1):
-        check(lib().lbm_set_launch_options(s.handle, k, int(rng.integers(0, 64))))
+        check(lib().lbm_set_launch_options(s.handle, k, int(rng.integers(0, 256))))
     load_state(s, o)
     nh = o.size_h
     for n in range(6):
