@@ -47,7 +47,8 @@ class Hip:
         for fn in ("hipEventCreate", "hipEventRecord", "hipEventSynchronize",
                    "hipEventElapsedTime", "hipEventDestroy", "hipStreamCreate",
                    "hipStreamSynchronize", "hipStreamDestroy", "hipDeviceSynchronize",
-                   "hipMemcpyAsync", "hipGetDeviceCount", "hipSetDevice"):
+                   "hipMemcpyAsync", "hipGetDeviceCount", "hipSetDevice", "hipHostMalloc",
+                   "hipHostFree"):
             getattr(self.L, fn).restype = ctypes.c_int
 
     def ok(self, rc, what):
@@ -75,6 +76,16 @@ class Hip:
 
     def sync(self, s):
         self.ok(self.L.hipStreamSynchronize(s), "hipStreamSynchronize")
+
+    def host_alloc(self, nbytes):
+        """Pinned host memory."""
+        p = ctypes.c_void_p()
+        self.ok(self.L.hipHostMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes), 0),
+                "hipHostMalloc")
+        return p
+
+    def host_free(self, p):
+        self.ok(self.L.hipHostFree(p), "hipHostFree")
 
 
 def measured_traffic(kernel, precision, size):
@@ -390,6 +401,10 @@ def main():
     ap.add_argument("--unpack", choices=("auto", "pair", "separate"), default="auto",
                     help="x-face unpack: both faces in one call (pair) or one call each; "
                          "auto = pair for 16-bit storage, separate otherwise")
+    ap.add_argument("--host-exchange", action="store_true",
+                    help="step: faces pass through pinned host buffers with the reference's "
+                         "calls (getDensityDistributions / setDensityDistributions), "
+                         "the PCIe-inclusive rate")
     ap.add_argument("--no-fused-pack", action="store_true",
                     help="pack the x faces with separate kernels instead of inside the steps")
     ap.add_argument("--cpu-sample-steps", type=int, default=2)
@@ -472,7 +487,12 @@ def main():
             check(L.lbm_device_alloc(device, solver.face_elements(d) * elem, ctypes.byref(p)))
             bufs.append(p)
 
-    fused = halo and not args.no_fused_pack
+    host_ex = halo and args.host_exchange
+    hbufs = []
+    if host_ex:
+        elem = 8 if args.precision == "f64" else 4
+        hbufs = [hip.host_alloc(solver.face_elements(d) * elem) for d in range(6)]
+    fused = halo and not args.no_fused_pack and not host_ex
     if fused:
         # the step kernels pack the LEFT/RIGHT faces themselves
         check(L.lbm_set_step_pack(solver.handle, bufs[0], bufs[1]))
@@ -483,7 +503,31 @@ def main():
     separate_unpack = (args.unpack == "separate" or
                        (args.unpack == "auto" and args.precision in ("f32", "f64")))
 
+    def box3(box):
+        return (ctypes.c_int * 3)(*box[0]), (ctypes.c_int * 3)(*box[1])
+
+    def host_halo_exchange(phase):
+        # the reference's calls on host buffers (CController.cpp:369-570,
+        # 807-1008): after alpha the offset-1 plane goes into the neighbour's
+        # halo plane, after beta the halo plane into its offset-1 plane, only
+        # the inward directions; every call returns with its copy complete
+        for axis in range(3):
+            for d in (2 * axis, 2 * axis + 1):
+                o, z = box3(solver.face_box(d, 0 if phase == 0 else 1))
+                check(L.lbm_get_density_distributions(solver.handle, o, z, hbufs[d], 0, stream))
+            for d in (2 * axis, 2 * axis + 1):
+                r = d ^ 1
+                o, z = box3(shadow.face_box(r, 1 if phase == 0 else 0))
+                if phase == 0:
+                    check(L.lbm_set_density_distributions(shadow.handle, o, z, hbufs[d], 0,
+                                                          stream))
+                else:
+                    check(L.lbm_set_density_distributions_dir(shadow.handle, o, z, r, hbufs[d],
+                                                              0, stream))
+
     def halo_exchange(phase):
+        if host_ex:
+            return host_halo_exchange(phase)
         # reference order: x faces, then y, then z (CController.cpp:369-570)
         for axis in range(3):
             if not (fused and axis == 0):
@@ -559,6 +603,8 @@ def main():
         shadow.close()
     for p in bufs:
         L.lbm_device_free(device, p)
+    for p in hbufs:
+        hip.host_free(p)
 
     t_elem = 8 if args.precision == "f64" else 4
     line = {
@@ -578,14 +624,17 @@ def main():
             "workload": (f"lid-driven cavity, D3Q19 BGK A-A alpha/beta, {size[0]}^3 interior "
                          f"({size[0] + 2}^3 with walls), {args.precision} populations, "
                          f"Re {args.reynolds:g}, lid {args.u_lid:g}"
-                         + (", six-face pack+unpack on the device every half-step" if halo else "")),
+                         + ((", six-face exchange through pinned host buffers every half-step"
+                             if host_ex else ", six-face pack+unpack on the device every half-step")
+                            if halo else "")),
             "size": list(size),
             "reynolds": args.reynolds,
             "u_lid": args.u_lid,
             "tau": round(params["tau"], 6),
             "precision": args.precision,
             "halo_exchange": halo,
-            "fused_x_pack": fused, "paired_unpack": halo and not separate_unpack,
+            "fused_x_pack": fused, "paired_unpack": halo and not separate_unpack and not host_ex,
+            "host_exchange": host_ex,
             "parallelism": "single GPU (one rank's subdomain)",
             "launch_options": {"alpha": launch_opts[0], "beta": launch_opts[1]},
         },
