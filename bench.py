@@ -469,11 +469,6 @@ def main():
     for k, opt in ((0, args.launch_alpha), (1, args.launch_beta)):
         if opt is not None:
             check(L.lbm_set_launch_options(solver.handle, k, opt))
-    launch_opts = []
-    for k in (0, 1):
-        v = ctypes.c_int()
-        check(L.lbm_get_launch_options(solver.handle, k, ctypes.byref(v)))
-        launch_opts.append(v.value)
     halo = not args.no_halo
     shadow = None
     bufs = []
@@ -496,6 +491,12 @@ def main():
     if fused:
         # the step kernels pack the LEFT/RIGHT faces themselves
         check(L.lbm_set_step_pack(solver.handle, bufs[0], bufs[1]))
+    # as launched (defaults may depend on the step pack)
+    launch_opts = []
+    for k in (0, 1):
+        v = ctypes.c_int()
+        check(L.lbm_get_launch_options(solver.handle, k, ctypes.byref(v)))
+        launch_opts.append(v.value)
 
     # Measured at 512^3: one call for both x faces halves the cache lines the
     # halo-plane unpack dirties, which pays for 16-bit storage; with FP32 the

@@ -86,6 +86,7 @@ struct lbm_solver {
     void* residual_prev;  // momentum saved by lbm_residual, 3 planes of the compute type
     int residual_valid;
     int launch[2];     // LBM_LAUNCH_* options of the alpha and beta kernels
+    int launch_set[2]; // set by the caller: kept when the step pack changes
     void* xpack[2];    // fused x-face pack buffers (LEFT, RIGHT), NULL when off
     void* dd_alt;      // second population buffer of the out-of-place pair pass
 };
@@ -137,13 +138,17 @@ constexpr int kReduceBlocks = 1024;
 // measured on MI355X at 512^3 with the six-face exchange: FP32 and FP64 gain
 // from fewer resident waves (fewer concurrent plane streams), 16-bit storage
 // needs every wave; FP32 beta from 512-thread workgroups of two cells per
-// thread, F64 beta from line-aligned accesses staged through LDS.
-inline int default_launch(int precision, int kernel)
+// thread, F64 beta from line-aligned accesses staged through LDS. FP32 alpha
+// also runs 512-thread workgroups unless it packs the x faces (with_pack):
+// the fused pack stores cost those workgroups more than they gain.
+inline int default_launch(int precision, int kernel, bool with_pack)
 {
     switch (precision) {
     case LBM_F32:
-        return kernel == 0 ? (LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_WAVES3)
-                           : (LBM_LAUNCH_WIDE | LBM_LAUNCH_WAVES2);
+        if (kernel == 0)
+            return with_pack ? (LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_WAVES3)
+                             : (LBM_LAUNCH_WIDE | LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_WAVES2);
+        return LBM_LAUNCH_WIDE | LBM_LAUNCH_WAVES2;
     case LBM_F64:
         return kernel == 0 ? (LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_WAVES2)
                            : (LBM_LAUNCH_LDS | LBM_LAUNCH_WAVES2);

@@ -163,8 +163,9 @@ int lbm_simulation_step_beta(lbm_solver* solver, const int origin[3], const int 
 
 /* Launch variants of the step kernels; they change speed, never results.
  * kernel: 0 = alpha, 1 = beta. Defaults (measured on MI355X at 512^3 with the
- * six-face exchange): F32 alpha NONTEMPORAL | WAVES3, beta WIDE | WAVES2; F64 alpha
- * NONTEMPORAL | WAVES2, beta LDS | WAVES2; F16S / BF16S alpha
+ * six-face exchange): F32 alpha NONTEMPORAL | WAVES3 with a step pack
+ * (lbm_set_step_pack), WIDE | NONTEMPORAL | WAVES2 without, beta WIDE | WAVES2;
+ * F64 alpha NONTEMPORAL | WAVES2, beta LDS | WAVES2; F16S / BF16S alpha
  * NONTEMPORAL, beta none. No reference
  * counterpart: the reference takes its block shapes from the XML
  * configuration (default.xml:93-115). */
@@ -287,7 +288,9 @@ int lbm_unpack_face_pair(lbm_solver* solver, int axis, int phase, const void* lo
  *          lbm_unpack_face(phase 1) reads (the other 14 are not written).
  * The x faces come first in the reference's exchange order
  * (CController.cpp:369-436, 807-874), so packing them inside the step is
- * exact; y and z faces keep lbm_pack_face. No reference counterpart. */
+ * exact; y and z faces keep lbm_pack_face. Launch options the caller has not
+ * set follow the precision's defaults for packing or not packing steps.
+ * No reference counterpart. */
 int lbm_set_step_pack(lbm_solver* solver, void* left_buf, void* right_buf);
 
 /* Single-GPU stand-in for the neighbour send/recv of one face: packs `from`'s

@@ -57,7 +57,7 @@ def test_every_launch_option_gives_the_same_bits(hip, precision):
 
 
 # per-precision defaults (alpha, beta) documented in include/lbm_abi.h
-DEFAULTS = {"f32": (1 | 8, 64 | 16), "f64": (1 | 16, 128 | 16), "f16s": (1, 0), "bf16s": (1, 0)}
+DEFAULTS = {"f32": (64 | 1 | 16, 64 | 16), "f64": (1 | 16, 128 | 16), "f16s": (1, 0), "bf16s": (1, 0)}
 
 
 @pytest.mark.parametrize("precision", sorted(DEFAULTS))
@@ -72,7 +72,22 @@ def test_default_options_are_valid_and_settable(hip, precision):
         assert v.value == DEFAULTS[precision][k]
     with pytest.raises(LbmError):
         check(lib().lbm_set_launch_options(s.handle, 0, 256))
+    # with a step pack FP32 alpha takes the 256-thread default; options the
+    # caller set are kept; without the pack the first defaults return
+    want_pack = {"f32": (1 | 8, 64 | 16)}.get(precision, DEFAULTS[precision])
+    buf = ctypes.c_void_p()
+    check(lib().lbm_device_alloc(0, s.face_elements(0) * 8, ctypes.byref(buf)))
+    check(lib().lbm_set_step_pack(s.handle, buf, None))
+    for k in (0, 1):
+        check(lib().lbm_get_launch_options(s.handle, k, ctypes.byref(v)))
+        assert v.value == want_pack[k]
+    check(lib().lbm_set_launch_options(s.handle, 1, 2))
+    check(lib().lbm_set_step_pack(s.handle, None, None))
+    for k, want in ((0, DEFAULTS[precision][0]), (1, 2)):
+        check(lib().lbm_get_launch_options(s.handle, k, ctypes.byref(v)))
+        assert v.value == want
     s.close()
+    check(lib().lbm_device_free(0, buf))
 
 
 @pytest.mark.parametrize("precision", ["f32", "f16s"])
