@@ -353,8 +353,8 @@ int lbm_create(const lbm_params* params, lbm_solver** out)
     s->stride = (n + 255) / 256 * 256;
     s->elem_bytes = eb;
     s->out_bytes = p.precision == LBM_F64 ? 8 : 4;
-    s->launch[0] = default_launch(p.precision, 0, false);
-    s->launch[1] = default_launch(p.precision, 1, false);
+    s->launch[0] = default_launch(p.precision, 0, false, s->n >= kWideCells);
+    s->launch[1] = default_launch(p.precision, 1, false, s->n >= kWideCells);
     s->launch_set[0] = s->launch_set[1] = 0;
     // round inv_tau / lid / gravity to the arithmetic type once
     if (p.precision != LBM_F64) {
@@ -777,7 +777,7 @@ int lbm_set_step_pack(lbm_solver* s, void* left_buf, void* right_buf)
     const bool with_pack = left_buf || right_buf;
     for (int k = 0; k < 2; k++)
         if (!s->launch_set[k])
-            s->launch[k] = default_launch(s->p.precision, k, with_pack);
+            s->launch[k] = default_launch(s->p.precision, k, with_pack, s->n >= kWideCells);
     return LBM_OK;
 }
 

@@ -552,10 +552,11 @@ int lbm_batch_create(lbm_solver* const* members, int count, lbm_batch** out)
     t.block = 256;
     t.cells_per_thread = 1;
     t.waves_per_cu = 0;
-    // the single-domain defaults measured with the exchange; the batch kernels
-    // take their shape from the tuning (block, cells_per_thread)
-    t.options[0] = default_launch(b->precision, 0, true) & kBatchOptions;
-    t.options[1] = default_launch(b->precision, 1, true) & kBatchOptions;
+    // the single-domain defaults of 256-thread workgroups measured with the
+    // exchange; the batch kernels take their shape from the tuning (block,
+    // cells_per_thread)
+    t.options[0] = default_launch(b->precision, 0, true, false) & kBatchOptions;
+    t.options[1] = default_launch(b->precision, 1, true, false) & kBatchOptions;
     t.pairs = 0;
     t.group_mb = 0;
     if ((rc = apply_tuning(b, t))) {

@@ -135,20 +135,26 @@ hipError_t launch_residual_members(int precision, const ResidualMember* mem, int
 constexpr int kReduceBlocks = 1024;
 
 // Default launch options of the step kernels (kernel 0 = alpha, 1 = beta),
-// measured on MI355X at 512^3 with the six-face exchange: FP32 and FP64 gain
-// from fewer resident waves (fewer concurrent plane streams), 16-bit storage
-// needs every wave; FP32 beta from 512-thread workgroups of two cells per
-// thread, F64 beta from line-aligned accesses staged through LDS. FP32 alpha
-// also runs 512-thread workgroups unless it packs the x faces (with_pack):
-// the fused pack stores cost those workgroups more than they gain.
-inline int default_launch(int precision, int kernel, bool with_pack)
+// measured on MI355X: FP32 and FP64 gain from fewer resident waves (fewer
+// concurrent plane streams), 16-bit storage needs every wave; F64 beta from
+// line-aligned accesses staged through LDS. FP32 from 512-thread workgroups
+// of two cells per thread on large domains (wide: at 192^3 and up; below
+// about 160^3 they leave too few workgroups), for alpha only when it does
+// not pack the x faces (with_pack: the fused pack stores cost those
+// workgroups more than they gain; FP32 alpha then keeps 3 waves per SIMD,
+// the best with the six-face exchange at 512^3).
+constexpr int64_t kWideCells = 6 << 20;   // halo-inclusive cells for `wide`
+
+inline int default_launch(int precision, int kernel, bool with_pack, bool wide)
 {
     switch (precision) {
     case LBM_F32:
-        if (kernel == 0)
-            return with_pack ? (LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_WAVES3)
-                             : (LBM_LAUNCH_WIDE | LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_WAVES2);
-        return LBM_LAUNCH_WIDE | LBM_LAUNCH_WAVES2;
+        if (kernel == 0) {
+            if (with_pack)
+                return LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_WAVES3;
+            return (wide ? LBM_LAUNCH_WIDE : 0) | LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_WAVES2;
+        }
+        return (wide ? LBM_LAUNCH_WIDE : 0) | LBM_LAUNCH_WAVES2;
     case LBM_F64:
         return kernel == 0 ? (LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_WAVES2)
                            : (LBM_LAUNCH_LDS | LBM_LAUNCH_WAVES2);

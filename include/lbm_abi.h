@@ -162,11 +162,12 @@ int lbm_simulation_step_beta(lbm_solver* solver, const int origin[3], const int 
                              hipStream_t stream);
 
 /* Launch variants of the step kernels; they change speed, never results.
- * kernel: 0 = alpha, 1 = beta. Defaults (measured on MI355X at 512^3 with the
- * six-face exchange): F32 alpha NONTEMPORAL | WAVES3 with a step pack
- * (lbm_set_step_pack), WIDE | NONTEMPORAL | WAVES2 without, beta WIDE | WAVES2;
- * F64 alpha NONTEMPORAL | WAVES2, beta LDS | WAVES2; F16S / BF16S alpha
- * NONTEMPORAL, beta none. No reference
+ * kernel: 0 = alpha, 1 = beta. Defaults (measured on MI355X): F32 beta
+ * WIDE | WAVES2 on domains of 6 Mi cells or more (halo included: from 183^3
+ * interior cells) and WAVES2 below; F32 alpha NONTEMPORAL | WAVES3 with a step
+ * pack (lbm_set_step_pack), else NONTEMPORAL | WAVES2, plus WIDE on those
+ * large domains; F64 alpha NONTEMPORAL | WAVES2, beta LDS | WAVES2; F16S /
+ * BF16S alpha NONTEMPORAL, beta none. No reference
  * counterpart: the reference takes its block shapes from the XML
  * configuration (default.xml:93-115). */
 enum lbm_launch_option {

@@ -57,7 +57,7 @@ def test_every_launch_option_gives_the_same_bits(hip, precision):
 
 
 # per-precision defaults (alpha, beta) documented in include/lbm_abi.h
-DEFAULTS = {"f32": (64 | 1 | 16, 64 | 16), "f64": (1 | 16, 128 | 16), "f16s": (1, 0), "bf16s": (1, 0)}
+DEFAULTS = {"f32": (1 | 16, 16), "f64": (1 | 16, 128 | 16), "f16s": (1, 0), "bf16s": (1, 0)}
 
 
 @pytest.mark.parametrize("precision", sorted(DEFAULTS))
@@ -72,9 +72,9 @@ def test_default_options_are_valid_and_settable(hip, precision):
         assert v.value == DEFAULTS[precision][k]
     with pytest.raises(LbmError):
         check(lib().lbm_set_launch_options(s.handle, 0, 256))
-    # with a step pack FP32 alpha takes the 256-thread default; options the
-    # caller set are kept; without the pack the first defaults return
-    want_pack = {"f32": (1 | 8, 64 | 16)}.get(precision, DEFAULTS[precision])
+    # with a step pack FP32 alpha keeps 3 waves per SIMD; options the caller
+    # set are kept; without the pack the first defaults return
+    want_pack = {"f32": (1 | 8, 16)}.get(precision, DEFAULTS[precision])
     buf = ctypes.c_void_p()
     check(lib().lbm_device_alloc(0, s.face_elements(0) * 8, ctypes.byref(buf)))
     check(lib().lbm_set_step_pack(s.handle, buf, None))
@@ -88,6 +88,21 @@ def test_default_options_are_valid_and_settable(hip, precision):
         assert v.value == want
     s.close()
     check(lib().lbm_device_free(0, buf))
+
+
+def test_large_fp32_domains_default_to_wide_workgroups(hip):
+    """From 6 Mi cells (halo included) FP32 defaults to 512-thread
+    workgroups: beta always, alpha while no step pack is set."""
+    import ctypes
+    from cavityflow_cuda_amd._lib import check, lib
+    v = ctypes.c_int()
+    for size, want in (((182, 182, 182), (1 | 16, 16)), ((183, 183, 183), (64 | 1 | 16, 64 | 16))):
+        s = hip.LbmSolverHIP(size, precision="f32", store_densities=False,
+                             store_velocities=False)
+        for k in (0, 1):
+            check(lib().lbm_get_launch_options(s.handle, k, ctypes.byref(v)))
+            assert v.value == want[k], (size, k)
+        s.close()
 
 
 @pytest.mark.parametrize("precision", ["f32", "f16s"])
