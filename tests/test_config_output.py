@@ -6,7 +6,6 @@ import ctypes
 import json
 import math
 import os
-import shutil
 
 import numpy as np
 import pytest

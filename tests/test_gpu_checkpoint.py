@@ -1,7 +1,6 @@
 """Checkpoint of the whole A-A state (lbm_save_state / lbm_load_state): a run
 resumed from a checkpoint is bit-identical to the uninterrupted run, for
 every storage precision and at both step parities."""
-import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu

@@ -2,7 +2,6 @@
 device-footprint query and admission-time rejections."""
 import math
 
-import numpy as np
 import pytest
 
 
