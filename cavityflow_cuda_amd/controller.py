@@ -63,6 +63,7 @@ class CavityController:
         self.simulationStepCounter = 0
         self.solvers = {}
         self.writers = {}
+        self.global_writer = None
         self.config = None
         px, py, pz = self.split
         for k, j, i in itertools.product(range(pz), range(py), range(px)):
@@ -91,15 +92,22 @@ class CavityController:
         """The cavity an XML configuration describes (``config.LbmConfiguration``):
         its size, subdomain split, CLbmSolver parameters and outputs, with a
         writer per rank when visualization is switched on: netCDF, the
-        reference's active writer (CController.cpp:120-135), or "vtk", the
+        reference's active writer (CController.cpp:120-135), "netcdf-global",
+        its parallel build's one file for the whole cavity, or "vtk", the
         writer it keeps commented out there."""
-        if vis_format not in ("netcdf", "vtk"):
-            raise ValueError(f"vis_format must be 'netcdf' or 'vtk', not {vis_format!r}")
+        if vis_format not in ("netcdf", "netcdf-global", "vtk"):
+            raise ValueError("vis_format must be 'netcdf', 'netcdf-global' or 'vtk', not "
+                             f"{vis_format!r}")
         c = cls(cfg.domain_size, precision=cfg.precision, store_outputs=cfg.store_outputs,
                 split=cfg.subdomains, device=device, params=cfg.solver_parameters(),
                 stream=stream)
         c.config = cfg
-        if cfg.do_visualization:
+        if cfg.do_visualization and vis_format == "netcdf-global":
+            from .output import NetCdfGlobalWriter
+            size, _, length = cfg.subdomain(0)
+            c.global_writer = NetCdfGlobalWriter(cfg.visualization_rate, cfg.visualization_dir,
+                                                 cfg.domain_size, size, length, cfg.precision)
+        elif cfg.do_visualization:
             from .output import NetCdfWriter, VtkWriter
             px, py, _ = cfg.subdomains
             for (i, j, k) in c.solvers:
@@ -116,8 +124,11 @@ class CavityController:
 
     def render(self, iteration: int) -> list:
         """CLbmVisualization::render of every rank (a file every `rate` iterations)."""
-        if self.writers:
+        if self.writers or self.global_writer:
             self.sync()
+        if self.global_writer:
+            p = self.global_writer.render(iteration, self)
+            return [p] if p else []
         return [p for key, w in self.writers.items()
                 if (p := w.render(iteration, self.solvers[key])) is not None]
 
@@ -175,7 +186,7 @@ class CavityController:
         t0 = time.perf_counter()
         for _ in range(loops):
             self.computeNextStep()
-            if self.writers:
+            if self.writers or self.global_writer:
                 self.render(self.simulationStepCounter)
         self.sync()
         elapsed = time.perf_counter() - t0
@@ -187,8 +198,8 @@ class CavityController:
                     glups=glups, mlups=glups * 1e3, bandwidth_gbs=glups * used)
 
     # -- extraction -----------------------------------------------------------
-    def _gather(self, getter, ncomp):
-        out = np.empty((ncomp,) + self.size[::-1], dtype=self.dtype)
+    def _gather(self, getter, ncomp, dtype=None):
+        out = np.empty((ncomp,) + self.size[::-1], dtype=dtype or self.dtype)
         sx, sy, sz = self.sub
         for (i, j, k), s in self.solvers.items():
             part = getattr(s, getter)().reshape(ncomp, sz, sy, sx)
@@ -201,3 +212,7 @@ class CavityController:
 
     def getDensities(self) -> np.ndarray:
         return self._gather("getDensities", 1)[0]
+
+    def getFlags(self) -> np.ndarray:
+        """Interior flags [z][y][x] of the whole cavity."""
+        return self._gather("getFlags", 1, np.int32)[0]

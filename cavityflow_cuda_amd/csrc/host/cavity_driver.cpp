@@ -5,7 +5,7 @@
 //   cavity_driver --size NX NY NZ [--split PX PY PZ] [--u-lid U] [--re RE]
 //                 [--loops N] [--precision f32|f64|f16s|bf16s] [--dump FILE]
 //   cavity_driver --config FILE.xml [--precision ...] [--loops N] [--dump FILE]
-//                 [--vis-format netcdf|vtk]
+//                 [--vis-format netcdf|netcdf-global|vtk]
 //   cavity_driver --config FILE.xml --print-config
 //
 // * --config reads the reference's XML configuration (CConfiguration.cpp):
@@ -14,8 +14,10 @@
 //   with visualization on, every rank writes the file of its interior at
 //   step 0 and after every step whose counter is a multiple of the rate
 //   (CController.cpp:1401-1430): netCDF by default, the reference's active
-//   writer (CController.cpp:120-135), or VTK, the one it keeps commented out
-//   (--vis-format vtk);
+//   writer (CController.cpp:120-135), one file per rank as its serial build
+//   or, with --vis-format netcdf-global, one file for the whole cavity as its
+//   parallel build; or VTK, the writer it keeps commented out (--vis-format
+//   vtk);
 // * --print-config prints the parsed fields and the derived parameters of
 //   rank 0 and exits without touching the GPU;
 // * with validation on and more than one subdomain, the same configuration is
@@ -195,6 +197,18 @@ public:
     {
         if (!cfg_ || !cfg_->do_visualization || iteration % cfg_->visualization_rate)
             return;
+        if (o_.vis_format == "netcdf-global") {
+            std::vector<T> u, rho;
+            std::vector<int32_t> flags;
+            gather(u, rho, &flags);
+            int size[3], origin[3];
+            double length[3];
+            cfg_->subdomain(0, size, origin, length);
+            const T len[3] = {(T)length[0], (T)length[1], (T)length[2]};
+            lbm::write_netcdf_global<T>(cfg_->visualization_dir, (int)iteration, o_.size, len,
+                                        size, flags.data(), rho.data(), u.data());
+            return;
+        }
         const long sc = (long)sub_[0] * sub_[1] * sub_[2];
         std::vector<T> rho(sc), u(3 * sc);
         std::vector<lbm::Flag> flags(sc);
@@ -227,19 +241,24 @@ public:
         return u;
     }
 
-    void gather(std::vector<T>& u, std::vector<T>& rho)
+    void gather(std::vector<T>& u, std::vector<T>& rho, std::vector<int32_t>* flags = nullptr)
     {
         const long nx = o_.size[0], ny = o_.size[1], nz = o_.size[2], cells = nx * ny * nz;
         u.assign(3 * cells, T(0));
         rho.assign(cells, T(0));
+        if (flags)
+            flags->assign(cells, 0);
         const long sc = (long)sub_[0] * sub_[1] * sub_[2];
         std::vector<T> pu(3 * sc), pr(sc);
+        std::vector<lbm::Flag> pf(flags ? sc : 0);
         for (int k = 0; k < o_.split[2]; k++)
             for (int j = 0; j < o_.split[1]; j++)
                 for (int i = 0; i < o_.split[0]; i++) {
                     auto* s = at(i, j, k);
                     s->getVelocities(pu.data());
                     s->getDensities(pr.data());
+                    if (flags)
+                        s->getFlags(pf.data());
                     for (int z = 0; z < sub_[2]; z++)
                         for (int y = 0; y < sub_[1]; y++)
                             for (int x = 0; x < sub_[0]; x++) {
@@ -249,6 +268,8 @@ public:
                                 for (int c = 0; c < 3; c++)
                                     u[c * cells + gid] = pu[c * sc + lid];
                                 rho[gid] = pr[lid];
+                                if (flags)
+                                    (*flags)[gid] = pf[lid];
                             }
                 }
     }
@@ -447,8 +468,9 @@ int main(int argc, char** argv)
         } else if (a == "--vis-format") {
             need(1);
             o.vis_format = argv[++i];
-            if (o.vis_format != "netcdf" && o.vis_format != "vtk") {
-                std::fprintf(stderr, "--vis-format takes netcdf or vtk\n");
+            if (o.vis_format != "netcdf" && o.vis_format != "netcdf-global" &&
+                o.vis_format != "vtk") {
+                std::fprintf(stderr, "--vis-format takes netcdf, netcdf-global or vtk\n");
                 return 2;
             }
         } else {

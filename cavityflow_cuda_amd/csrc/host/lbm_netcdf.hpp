@@ -1,6 +1,7 @@
-// netCDF output of one rank's interior, the reference's active writer
-// (CLbmVisualizationNetCDF.cpp:92-228), written without the netCDF library
-// in the classic format: big-endian header, then each variable's values.
+// netCDF output, the reference's active writer (CLbmVisualizationNetCDF.cpp:
+// 92-228), written without the netCDF library in the classic format:
+// big-endian header, then each variable's values. One file per rank (the
+// serial build) or one for the whole domain (the parallel build).
 //
 // Layout (as the reference defines it, CLbmVisualizationNetCDF.cpp:92-155):
 //   dimensions x, y, z (defined in that order: ids 0, 1, 2);
@@ -142,15 +143,16 @@ void put_be(std::ofstream& f, const V* values, uint64_t count)
 
 }  // namespace netcdf
 
-// visualization_<rank>_<iteration>.nc in `dir` (the reference's serial file
-// name, CLbmVisualizationNetCDF.cpp:55-56) for interior fields flags / rho
-// [z][y][x] and u [3][z][y][x] of a subdomain of `size` cells and physical
-// `length`. variant: 0 chooses the classic or 64-bit-offset form by size,
-// 1 / 2 force it.
+// One netCDF file of interior fields flags / rho [z][y][x] and u [3][z][y][x]
+// over `size` cells; the coordinate variables hold the cell centres
+// ((T)i + 0.5) * cell_length / (T)cell_size (CLbmVisualizationNetCDF.cpp:
+// 166-203: the subdomain's length and size, also for the whole-domain file of
+// the parallel build). variant: 0 chooses the classic or 64-bit-offset form
+// by size, 1 / 2 force it.
 template <class T>
-std::string write_netcdf(const std::string& dir, int rank, int iteration, const int size[3],
-                         const T length[3], const int32_t* flags, const T* rho, const T* u,
-                         int variant = 0)
+void write_netcdf_file(const std::string& path, const int size[3], const T cell_length[3],
+                       const int cell_size[3], const int32_t* flags, const T* rho, const T* u,
+                       int variant = 0)
 {
     using namespace netcdf;
     const int32_t tt = typeid(T) == typeid(double) ? kDouble : kFloat;
@@ -181,16 +183,14 @@ std::string write_netcdf(const std::string& dir, int rank, int iteration, const 
                                   : last_begin + 1024 > 0x7fffffffull || largest > (1ull << 31) - 4;
     const std::vector<uint8_t> head = header(size, vars, offset64);
 
-    std::stringstream name;
-    name << dir << "/visualization_" << rank << "_" << iteration << ".nc";
-    std::ofstream f(name.str().c_str(), std::ios::out | std::ios::binary | std::ios::trunc);
+    std::ofstream f(path.c_str(), std::ios::out | std::ios::binary | std::ios::trunc);
     if (!f)
-        throw std::runtime_error("cannot open " + name.str());
+        throw std::runtime_error("cannot open " + path);
     f.write((const char*)head.data(), (std::streamsize)head.size());
     for (int d = 0; d < 3; d++) {
         std::vector<T> c(size[d]);
-        for (int i = 0; i < size[d]; i++)     // CLbmVisualizationNetCDF.cpp:180-203
-            c[i] = ((T)i + (T)0.5) * length[d] / (T)size[d];
+        for (int i = 0; i < size[d]; i++)
+            c[i] = ((T)i + (T)0.5) * cell_length[d] / (T)cell_size[d];
         put_be(f, c.data(), (uint64_t)size[d]);
     }
     put_be(f, flags, cells);
@@ -198,7 +198,35 @@ std::string write_netcdf(const std::string& dir, int rank, int iteration, const 
     for (int k = 0; k < 3; k++)
         put_be(f, u + k * cells, cells);
     if (!f)
-        throw std::runtime_error("write failed: " + name.str());
+        throw std::runtime_error("write failed: " + path);
+}
+
+// visualization_<rank>_<iteration>.nc in `dir` (the serial build's file name,
+// CLbmVisualizationNetCDF.cpp:55-56) for one subdomain of `size` cells and
+// physical `length`.
+template <class T>
+std::string write_netcdf(const std::string& dir, int rank, int iteration, const int size[3],
+                         const T length[3], const int32_t* flags, const T* rho, const T* u,
+                         int variant = 0)
+{
+    std::stringstream name;
+    name << dir << "/visualization_" << rank << "_" << iteration << ".nc";
+    write_netcdf_file(name.str(), size, length, size, flags, rho, u, variant);
+    return name.str();
+}
+
+// visualization_<iteration>.nc in `dir` (the parallel build's file,
+// CLbmVisualizationNetCDF.cpp:52, 107-111): the whole domain of `size`
+// cells, each subdomain's fields at its global origin; coordinates from the
+// subdomain's length and size.
+template <class T>
+std::string write_netcdf_global(const std::string& dir, int iteration, const int size[3],
+                                const T sub_length[3], const int sub_size[3],
+                                const int32_t* flags, const T* rho, const T* u, int variant = 0)
+{
+    std::stringstream name;
+    name << dir << "/visualization_" << iteration << ".nc";
+    write_netcdf_file(name.str(), size, sub_length, sub_size, flags, rho, u, variant);
     return name.str();
 }
 

@@ -16,7 +16,8 @@ x, y, z; coordinate variables x(x), y(y), z(z) at the cell centres
 velocities_x / _y / _z over (z, y, x); a "units" attribute on each. Big-endian
 header, data right after it, variables in definition order, each padded to
 4 bytes; the 64-bit-offset variant (version byte 2) when an offset or a
-variable needs more than 31 bits.
+variable needs more than 31 bits. The parallel build's single file,
+``visualization_<iteration>.nc``, holds the whole cavity instead.
 """
 from __future__ import annotations
 
@@ -100,19 +101,23 @@ def _nc_text(s: str) -> bytes:
     return struct.pack(">i", len(b)) + b + b"\0" * (-len(b) % 4)
 
 
-def netcdf_bytes(size, length, flags, rho, u, precision="f32", offset64=None) -> bytes:
+def netcdf_bytes(size, length, flags, rho, u, precision="f32", offset64=None,
+                 cell_size=None) -> bytes:
     """The .nc file contents for interior fields flags / rho [z][y][x] and
-    u [3][z][y][x] of a subdomain of `size` cells with physical `length`
-    (CLbmVisualizationNetCDF.cpp:92-155 layout, 166-228 values).
+    u [3][z][y][x] over `size` cells (CLbmVisualizationNetCDF.cpp:92-155
+    layout, 166-228 values). The coordinates are the cell centres
+    ((T)i + 0.5) * length / (T)cell_size: `length` and `cell_size` are a
+    subdomain's (cell_size defaults to `size`, one subdomain's file).
     offset64: None chooses the format variant by size; True / False force it."""
     T = np.dtype(np.float64 if precision == "f64" else np.float32)
     tt = _NC_DOUBLE if T == np.float64 else _NC_FLOAT
     nx, ny, nz = (int(v) for v in size)
+    cn = tuple(int(v) for v in (cell_size if cell_size is not None else size))
     cells = nx * ny * nz
     coords = []
-    for a, n in enumerate((nx, ny, nz)):     # ((T)i + (T)0.5) * length / (T)size, in T
+    for a, n in enumerate((nx, ny, nz)):     # ((T)i + (T)0.5) * length / (T)cell_size, in T
         i = np.arange(n).astype(T)
-        coords.append((i + T.type(0.5)) * T.type(length[a]) / T.type(n))
+        coords.append((i + T.type(0.5)) * T.type(length[a]) / T.type(cn[a]))
     uu = np.asarray(u, dtype=T).reshape(3, cells)
     zyx = (2, 1, 0)
     vars_ = [("x", (0,), tt, "meters", coords[0].astype(T.newbyteorder(">"))),
@@ -178,4 +183,31 @@ class NetCdfWriter:
         with open(p, "wb") as f:
             f.write(netcdf_bytes(self.size, self.length, solver.getFlags(),
                                  solver.getDensities(), solver.getVelocities(), self.precision))
+        return p
+
+
+class NetCdfGlobalWriter:
+    """CLbmVisualizationNetCDF<T> of the parallel build: render(iteration)
+    writes visualization_<iteration>.nc every `rate` iterations with the whole
+    cavity, each subdomain's fields at its global origin
+    (CLbmVisualizationNetCDF.cpp:52, 107-111, 171-228), the coordinates from a
+    subdomain's length and size."""
+
+    def __init__(self, rate, directory, size, sub_size, sub_length, precision="f32"):
+        self.rate, self.directory = int(rate), directory
+        self.size, self.sub_size = tuple(size), tuple(sub_size)
+        self.sub_length = tuple(sub_length)
+        self.precision = precision
+
+    def path(self, iteration) -> str:
+        return os.path.join(self.directory, f"visualization_{iteration}.nc")
+
+    def render(self, iteration, controller):
+        if iteration % self.rate:
+            return None
+        p = self.path(iteration)
+        with open(p, "wb") as f:
+            f.write(netcdf_bytes(self.size, self.sub_length, controller.getFlags(),
+                                 controller.getDensities(), controller.getVelocities(),
+                                 self.precision, cell_size=self.sub_size))
         return p

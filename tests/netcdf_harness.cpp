@@ -1,8 +1,9 @@
 // Writes one netCDF file through lbm::write_netcdf from raw little-endian
 // inputs, for the byte comparison with the Python writer:
-//   netcdf_harness f32|f64 DIR RANK ITER NX NY NZ LX LY LZ VARIANT INPUT
+//   netcdf_harness f32|f64 DIR RANK ITER NX NY NZ LX LY LZ VARIANT INPUT [SX SY SZ]
 // INPUT holds flags (int32) [z][y][x], rho [z][y][x] and u [3][z][y][x] in the
-// arithmetic type, one after the other.
+// arithmetic type, one after the other. With SX SY SZ (a subdomain's size; LX
+// LY LZ are then its length) the whole-domain file of write_netcdf_global.
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -12,7 +13,7 @@
 #include "lbm_netcdf.hpp"
 
 template <class T>
-static int run(char** a)
+static int run(char** a, bool global)
 {
     const std::string dir = a[2];
     const int rank = std::atoi(a[3]), iter = std::atoi(a[4]);
@@ -28,14 +29,22 @@ static int run(char** a)
     in.read((char*)u.data(), 3 * cells * sizeof(T));
     if (!in)
         return 2;
-    std::printf("%s\n", lbm::write_netcdf<T>(dir, rank, iter, size, length, flags.data(),
-                                              rho.data(), u.data(), variant).c_str());
+    if (global) {
+        const int sub[3] = {std::atoi(a[13]), std::atoi(a[14]), std::atoi(a[15])};
+        std::printf("%s\n", lbm::write_netcdf_global<T>(dir, iter, size, length, sub,
+                                                         flags.data(), rho.data(), u.data(),
+                                                         variant).c_str());
+    } else {
+        std::printf("%s\n", lbm::write_netcdf<T>(dir, rank, iter, size, length, flags.data(),
+                                                  rho.data(), u.data(), variant).c_str());
+    }
     return 0;
 }
 
 int main(int argc, char** argv)
 {
-    if (argc != 13)
+    if (argc != 13 && argc != 16)
         return 1;
-    return std::string(argv[1]) == "f64" ? run<double>(argv) : run<float>(argv);
+    const bool global = argc == 16;
+    return std::string(argv[1]) == "f64" ? run<double>(argv, global) : run<float>(argv, global);
 }

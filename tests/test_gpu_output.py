@@ -111,6 +111,50 @@ def test_driver_and_controller_write_identical_netcdf(tmp_path, hip, precision):
 
 
 @pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_driver_and_controller_write_identical_global_netcdf(tmp_path, hip, precision):
+    """--vis-format netcdf-global / vis_format="netcdf-global": one file per
+    output step for the whole cavity (the reference's parallel build), equal
+    to the gathered fields of both subdomains."""
+    from scipy.io import netcdf_file
+    from cavityflow_cuda_amd import build
+    from cavityflow_cuda_amd.config import LbmConfiguration
+    print("[case] netcdf-global", precision, flush=True)
+    T = np.float64 if precision == "f64" else np.float32
+    exe = os.path.join(os.path.dirname(build.__file__), "cavity_driver")
+    cpp_dir, py_dir = tmp_path / "cpp", tmp_path / "py"
+    cpp_dir.mkdir()
+    py_dir.mkdir()
+    cfg_cpp = tmp_path / "cpp.xml"
+    cfg_cpp.write_text(config_text(str(cpp_dir)))
+    cfg_py = tmp_path / "py.xml"
+    cfg_py.write_text(config_text(str(py_dir)))
+    r = subprocess.run([exe, "--config", str(cfg_cpp), "--precision", precision,
+                        "--vis-format", "netcdf-global"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    cfg = LbmConfiguration.from_xml(str(cfg_py), precision)
+    c = hip.CavityController.from_config(cfg, vis_format="netcdf-global")
+    c.run(cfg.loops)
+    c.sync()
+    flags, rho, u = c.getFlags(), c.getDensities(), c.getVelocities()
+    c.close()
+    names = sorted(os.listdir(py_dir))
+    assert names == sorted(f"visualization_{i}.nc" for i in (0, 3, 6))
+    assert sorted(os.listdir(cpp_dir)) == names
+    for n in names:
+        assert (cpp_dir / n).read_bytes() == (py_dir / n).read_bytes(), n
+    f = netcdf_file(str(py_dir / "visualization_6.nc"), "r", mmap=False)
+    assert f.dimensions == {"x": 12, "y": 10, "z": 8}
+    assert f.variables["flags"].data.tolist() == flags.tolist()
+    assert f.variables["densities"].data.astype(T).tobytes() == np.asarray(rho, T).tobytes()
+    for k, name in enumerate(("velocities_x", "velocities_y", "velocities_z")):
+        assert f.variables[name].data.astype(T).tobytes() == np.asarray(u[k], T).tobytes()
+    x = f.variables["x"].data
+    assert abs(float(x[0]) - 0.05) < 1e-6 and abs(float(x[-1]) - 1.15) < 1e-6
+    f.close()
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
 def test_driver_validation_mode(tmp_path, hip, precision):
     """validation do=1 (main.cpp:121-303): the split run is compared cell by
     cell with a one-subdomain run of the same configuration."""

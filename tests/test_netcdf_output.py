@@ -109,3 +109,31 @@ def test_native_writer_bytes_equal_python_writer(tmp_path, harness, precision, v
     want = netcdf_bytes(size, tuple(T(v) for v in length), flags, rho, u, precision,
                         offset64={0: None, 1: False, 2: True}[variant])
     assert open(path, "rb").read() == want
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_native_global_file_bytes_equal_python_writer(tmp_path, harness, precision):
+    """The parallel build's whole-domain file: coordinates over the whole
+    extent from a subdomain's length and size."""
+    from cavityflow_cuda_amd.output import NetCdfGlobalWriter, netcdf_bytes
+    size, sub, sub_length = (12, 4, 10), (6, 4, 5), (0.3, 0.2, 0.25)
+    flags, rho, u = fields(size, precision, 9)
+    raw = tmp_path / "in.bin"
+    raw.write_bytes(flags.tobytes() + rho.tobytes() + u.tobytes())
+    T = np.float64 if precision == "f64" else np.float32
+    r = subprocess.run([harness, precision, str(tmp_path), "-1", "40", *map(str, size),
+                        *(repr(float(T(v))) for v in sub_length), "0", str(raw), *map(str, sub)],
+                       capture_output=True, text=True, check=True)
+    path = r.stdout.strip()
+    assert path == os.path.join(str(tmp_path), "visualization_40.nc")
+    assert path == NetCdfGlobalWriter(20, str(tmp_path), size, sub, sub_length, precision).path(40)
+    data = open(path, "rb").read()
+    assert data == netcdf_bytes(size, tuple(T(v) for v in sub_length), flags, rho, u, precision,
+                                cell_size=sub)
+    f = read(data)
+    assert f.dimensions == {"x": 12, "y": 4, "z": 10}
+    # cell length 0.3 / 6 = 0.05 on x: centres 0.025 ... 0.575 over both subdomains
+    for a, name in enumerate("xyz"):
+        want = [T(T(T(i) + T(0.5)) * T(sub_length[a])) / T(sub[a]) for i in range(size[a])]
+        assert f.variables[name].data.astype(T).tobytes() == np.array(want, T).tobytes()
+    f.close()
