@@ -428,7 +428,7 @@ def main():
                     help="serving: arrival rate as a fraction of measured capacity "
                          "(0: all requests arrive at once)")
     ap.add_argument("--hbm-budget", type=float, default=200.0, help="serving: GB")
-    ap.add_argument("--policy", default="srpt_oldest", choices=("srpt", "srpt_oldest", "all"),
+    ap.add_argument("--policy", default="hrrn", choices=("srpt", "srpt_oldest", "hrrn", "all"),
                     help="serving: which resident solves each round advances")
     args = ap.parse_args()
 

@@ -20,8 +20,12 @@ the policy: "srpt" takes them in order of least remaining work (cells x steps
 left to the cap) until the batch holds ``min_batch_cells`` cells, enough to
 fill the GPU, so short solves are not held behind long ones; "srpt_oldest"
 adds the longest-resident solve to that batch every round, so no solve
-waits behind an endless stream of shorter ones; "all" advances every
-resident solve every round (processor sharing). Admission is FIFO
+waits behind an endless stream of shorter ones; "hrrn" (highest response
+ratio next) orders them by (time since arrival + remaining work at the
+measured rate) / remaining work instead, so waiting raises a long solve's
+priority steadily (the default: at 80 % load it measured the lowest p50 and
+p99 of the three); "all" advances every resident solve every round
+(processor sharing). Admission is FIFO
 under an HBM budget: a request is admitted when its device footprint
 (``lbm_device_bytes``) fits in what the resident solves leave free; one that
 can never fit is rejected.
@@ -153,7 +157,7 @@ class CavityServer:
     """Continuous-batching solve server on one GPU (see module docstring)."""
 
     def __init__(self, precision="f32", hbm_budget=200e9, check_every=100, device=0,
-                 stream=None, tuning=None, max_members=4096, policy="srpt_oldest",
+                 stream=None, tuning=None, max_members=4096, policy="hrrn",
                  min_batch_cells=16_000_000):
         if check_every % 2 or check_every <= 0:
             raise ValueError("check_every must be even (whole alpha + beta pairs)")
@@ -164,8 +168,8 @@ class CavityServer:
         self.stream = stream
         self.tuning = tuning
         self.max_members = int(max_members)
-        if policy not in ("srpt", "srpt_oldest", "all"):
-            raise ValueError("policy is 'srpt', 'srpt_oldest' or 'all'")
+        if policy not in ("srpt", "srpt_oldest", "hrrn", "all"):
+            raise ValueError("policy is 'srpt', 'srpt_oldest', 'hrrn' or 'all'")
         self.policy = policy
         self.min_batch_cells = int(min_batch_cells)
         self._batch_ids: tuple = ()
@@ -248,7 +252,18 @@ class CavityServer:
         """The resident solves this round advances."""
         if self.policy == "all":
             return list(self.active)
-        order = sorted(self.active, key=lambda a: (a[0].cells * (a[0].max_steps - a[3]), a[4]))
+        if self.policy == "hrrn":
+            now = self.clock()
+            # cell updates per second so far (the first rounds: a nominal rate)
+            rate = self.cell_updates / now if self.cell_updates and now > 0 else 3e10
+
+            def ratio(a):
+                service = a[0].cells * (a[0].max_steps - a[3]) / rate
+                return (now - a[0].arrival + service) / max(service, 1e-9)
+            order = sorted(self.active, key=lambda a: (-ratio(a), a[4]))
+        else:
+            order = sorted(self.active,
+                           key=lambda a: (a[0].cells * (a[0].max_steps - a[3]), a[4]))
         chosen, cells = [], 0
         for a in order:
             chosen.append(a)

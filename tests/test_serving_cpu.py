@@ -63,3 +63,25 @@ def test_latency_summary():
     s = latency_summary(res)
     assert s["completed"] == 10 and s["p50_s"] == 1.0 and s["max_s"] == 2.0
     assert math.isclose(s["span_s"], 11.0) and math.isclose(s["solves_per_s"], 10 / 11.0)
+
+
+def test_hrrn_orders_by_response_ratio(hip):
+    """policy="hrrn": (waited + remaining work at the measured rate) /
+    remaining work, highest first; a long solve that has waited long enough
+    goes ahead of a short one that just arrived."""
+    from cavityflow_cuda_amd.serving import CavityServer, SolveRequest
+    srv = CavityServer("f32", policy="hrrn", min_batch_cells=10**12)
+    srv.clock = lambda: 10.0
+    srv.cell_updates = 10**9                     # 1e8 cell updates per second so far
+    big = SolveRequest(0, (100, 100, 100), 0.1, 100.0, arrival=0.0, max_steps=1000)
+    fresh = SolveRequest(1, (10, 10, 100), 0.1, 100.0, arrival=9.95, max_steps=1000)
+    waited = SolveRequest(2, (10, 10, 100), 0.1, 100.0, arrival=9.0, max_steps=1000)
+    # [request, solver, bytes, steps done, admitted]; the solver is not used
+    srv.active = [[big, None, 0, 0, 0.0], [fresh, None, 0, 0, 9.95], [waited, None, 0, 0, 9.0]]
+    # ratios: waited (1 + 0.1) / 0.1 = 11, big (10 + 10) / 10 = 2, fresh (0.05 + 0.1) / 0.1 = 1.5
+    assert [a[0].id for a in srv._select()] == [2, 0, 1]
+    srv.min_batch_cells = 1
+    assert [a[0].id for a in srv._select()] == [2]
+    srv.active = []
+    with pytest.raises(ValueError):
+        CavityServer("f32", policy="fifo")
