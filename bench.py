@@ -317,11 +317,11 @@ def serving_workload(args, cf, L, hip, stream):
     sample = [CavitySpec(r.size, r.u_lid, r.reynolds, r.params) for r in reqs[:24]]
     b = CavityBatch(sample, "f32")
     tuning = b.autotune(trial_steps=8, stream=stream.value)
-    b.run(100, stream=stream.value)
+    b.run(100, stream=stream.value, graph=not args.no_graph)
     hip.sync(stream)
     e0, e1 = hip.event(), hip.event()
     hip.record(e0, stream)
-    b.run(300, stream=stream.value)
+    b.run(300, stream=stream.value, graph=not args.no_graph)
     hip.record(e1, stream)
     cal = b.cells * 300 / (hip.elapsed_ms(e0, e1) * 1e-3)
     b.close()
@@ -340,7 +340,8 @@ def serving_workload(args, cf, L, hip, stream):
               flush=True)
 
     srv = sv.CavityServer("f32", hbm_budget=args.hbm_budget * 1e9, check_every=100,
-                          stream=stream.value, tuning=tuning, policy=args.policy)
+                          stream=stream.value, tuning=tuning, policy=args.policy,
+                          graph=not args.no_graph)
     served = srv.run(reqs, progress=prog)
     peak_gb = srv.peak_used / 1e9
     rounds = srv.rounds
@@ -375,6 +376,7 @@ def serving_workload(args, cf, L, hip, stream):
                                f"(aspect <= 2), Re U[100, 3200], lid 0.1, residual 1e-6 or "
                                f"{args.max_steps} steps, HBM budget {args.hbm_budget:.0f} GB, "
                                f"seed {args.seed}, policy {args.policy}", "redrawn": redrawn,
+                   "graph_replay": not args.no_graph,
                    "arrival_rate_per_s": round(rate, 4) if math.isfinite(rate) else None},
         "served": dict(s_sum, aggregate_mlups=round(s_mlups, 1), peak_resident_gb=round(peak_gb, 2),
                        rounds=rounds, statuses=statuses),
@@ -419,8 +421,8 @@ def main():
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--cavities", type=int, default=64, help="batched: number of cavities")
     ap.add_argument("--no-graph", action="store_true",
-                    help="batched: issue the batch launches directly instead of replaying "
-                         "captured graphs")
+                    help="batched / serving: issue the batch launches directly instead of "
+                         "replaying captured graphs")
     ap.add_argument("--requests", type=int, default=200, help="serving: number of requests")
     ap.add_argument("--max-steps", type=int, default=2000,
                     help="serving: time-step cap per request")

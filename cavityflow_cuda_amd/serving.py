@@ -158,7 +158,7 @@ class CavityServer:
 
     def __init__(self, precision="f32", hbm_budget=200e9, check_every=100, device=0,
                  stream=None, tuning=None, max_members=4096, policy="hrrn",
-                 min_batch_cells=16_000_000):
+                 min_batch_cells=16_000_000, graph=True):
         if check_every % 2 or check_every <= 0:
             raise ValueError("check_every must be even (whole alpha + beta pairs)")
         self.precision = precision
@@ -172,6 +172,7 @@ class CavityServer:
             raise ValueError("policy is 'srpt', 'srpt_oldest', 'hrrn' or 'all'")
         self.policy = policy
         self.min_batch_cells = int(min_batch_cells)
+        self.graph = bool(graph)        # False: the round's launches issued directly
         self._batch_ids: tuple = ()
         self.queue: deque = deque()
         self.active: list = []          # [req, solver, bytes, steps, admitted]
@@ -325,7 +326,8 @@ class CavityServer:
             self._rebuild(members)
             self._dirty = False
         st = _stream(self.stream)
-        check(lib().lbm_batch_run(self._batch, 0, self.K, st))
+        run = lib().lbm_batch_run if self.graph else lib().lbm_batch_launch
+        check(run(self._batch, 0, self.K, st))
         res = (ctypes.c_double * len(members))()
         check(lib().lbm_batch_residual(self._batch, 0, res, st))
         self.rounds += 1

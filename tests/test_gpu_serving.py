@@ -30,14 +30,15 @@ def serve_staggered(srv, reqs):
 PAIRS = dict(block=256, cells_per_thread=1, waves_per_cu=0, options=(2, 0), pairs=1)
 
 
-@pytest.mark.parametrize("pairs", [0, 1])
+@pytest.mark.parametrize("pairs,graph", [(0, True), (1, True), (0, False)])
 @pytest.mark.parametrize("precision", ["f32", "f16s"])
-def test_results_same_alone_and_in_the_mix(hip, precision, pairs):
+def test_results_same_alone_and_in_the_mix(hip, precision, pairs, graph):
     from cavityflow_cuda_amd.serving import CavityServer, solve_sequential
-    print("[case] serving mix", precision, "pairs", pairs, flush=True)
+    print("[case] serving mix", precision, "pairs", pairs, "graph", graph, flush=True)
     # tol 2e-4 lets the small, viscous cavities converge early, the others hit the cap;
-    # the solo runs always use the in-place half-step launches
-    srv = CavityServer(precision, check_every=52, tuning=PAIRS if pairs else None)
+    # the solo runs always use the in-place half-step launches; graph=False issues
+    # each round's launches directly
+    srv = CavityServer(precision, check_every=52, tuning=PAIRS if pairs else None, graph=graph)
     mixed = serve_staggered(srv, requests(tol=2e-4, max_steps=624))
     alone = solve_sequential(requests(tol=2e-4, max_steps=624), precision, check_every=52)
     statuses = set()
