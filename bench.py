@@ -253,7 +253,7 @@ def batched_workload(args, cf, L, hip, stream):
         for sp in specs:
             sp.params = cf.cavity_parameters(sp.size, sp.u_lid, sp.reynolds, prec)
         b = CavityBatch(specs, prec)
-        tuning = b.autotune(trial_steps=8, stream=stream.value)
+        tuning = b.autotune(trial_steps=args.trial_steps, stream=stream.value)
         print(f"[batched] {prec} tuning {tuning}", file=sys.stderr, flush=True)
         b.run(100, stream=stream.value, graph=not args.no_graph)   # warm-up (graph capture)
         hip.sync(stream)
@@ -316,7 +316,7 @@ def serving_workload(args, cf, L, hip, stream):
     # capacity: the batch rate on a sample of the requests, with autotuned shapes
     sample = [CavitySpec(r.size, r.u_lid, r.reynolds, r.params) for r in reqs[:24]]
     b = CavityBatch(sample, "f32")
-    tuning = b.autotune(trial_steps=8, stream=stream.value)
+    tuning = b.autotune(trial_steps=args.trial_steps, stream=stream.value)
     b.run(100, stream=stream.value, graph=not args.no_graph)
     hip.sync(stream)
     e0, e1 = hip.event(), hip.event()
@@ -423,6 +423,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="batched / serving: issue the batch launches directly instead of "
                          "replaying captured graphs")
+    ap.add_argument("--trial-steps", type=int, default=8,
+                    help="batched / serving: time steps per autotuner trial")
     ap.add_argument("--requests", type=int, default=200, help="serving: number of requests")
     ap.add_argument("--max-steps", type=int, default=2000,
                     help="serving: time-step cap per request")

@@ -775,6 +775,19 @@ int lbm_batch_autotune(lbm_batch* b, int trial_steps, hipStream_t stream,
                 return rc;
         }
     }
+    // with the state of a group in the cache the best workgroup shape can
+    // differ from the streaming one: the shapes once more, grouped
+    if (best.group_mb > 0) {
+        const lbm_batch_tuning grouped = best;
+        for (int blk : {256, 512})
+            for (int cpt : {1, 2, 4}) {
+                lbm_batch_tuning cand = grouped;
+                cand.block = blk;
+                cand.cells_per_thread = cpt;
+                if ((rc = consider(cand)))
+                    return rc;
+            }
+    }
     // the pair pass, when the members allow it
     bool eligible = true;
     for (int shape = 1; shape <= kPairShapes && eligible; shape++)

@@ -371,9 +371,10 @@ int lbm_batch_run(lbm_batch* batch, int first_phase, int half_steps, hipStream_t
  * capture (same schedule, same bits). */
 int lbm_batch_launch(lbm_batch* batch, int first_phase, int half_steps, hipStream_t stream);
 /* Times candidate tunings (block, cells_per_thread, waves_per_cu, launch
- * options, member groups, and the pair pass when the members allow it) as
- * runs of trial_steps whole steps (graph-captured on a stream) and keeps the
- * fastest; the winner must also beat the starting tuning over longer runs.
+ * options, member groups with the shapes again under the chosen group, and
+ * the pair pass when the members allow it) as runs of trial_steps whole steps
+ * (graph-captured on a stream) and keeps the fastest; the winner must also
+ * beat the starting tuning over longer runs.
  * The trials are real steps: the members advance; reset them afterwards
  * (lbm_reset) when their state matters. Synchronous. */
 int lbm_batch_autotune(lbm_batch* batch, int trial_steps, hipStream_t stream,
