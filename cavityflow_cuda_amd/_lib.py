@@ -20,6 +20,7 @@ LAUNCH_ADDR64 = 32
 LAUNCH_WIDE = 64
 LAUNCH_LDS = 128
 LBM_OK = 0
+NOALLOC_MAX_BOXES = 16  # LBM_NOALLOC_MAX_BOXES
 (LBM_ERR_ARG, LBM_ERR_BOX, LBM_ERR_HIP, LBM_ERR_NOMEM, LBM_ERR_PARAMS, LBM_ERR_STATE,
  LBM_ERR_UNSUPPORTED) = range(-1, -8, -1)
 
@@ -38,7 +39,7 @@ EXPORTED = (
     "lbm_batch_autotune",
     "lbm_batch_residual", "lbm_reset", "lbm_device_bytes", "lbm_state_bytes",
     "lbm_save_state", "lbm_load_state",
-    "lbm_reduce_box", "lbm_residual", "lbm_sync", "lbm_last_error", "lbm_status_string",
+    "lbm_reduce_box", "lbm_reduce_boxes_noalloc", "lbm_residual", "lbm_sync", "lbm_last_error", "lbm_status_string",
     "lbm_abi_version",
 )
 
@@ -177,6 +178,8 @@ def lib() -> ctypes.CDLL:
         "lbm_load_state": ([P, P, ctypes.c_uint64], I),
         "lbm_device_bytes": ([ctypes.POINTER(lbm_params), I, ctypes.POINTER(ctypes.c_uint64)], I),
         "lbm_reduce_box": ([P, i3, i3, ctypes.POINTER(ctypes.c_double), P], I),
+        "lbm_reduce_boxes_noalloc": ([ctypes.POINTER(P), ctypes.POINTER(ctypes.c_int),
+                                      ctypes.POINTER(ctypes.c_int), I, P, P], I),
         "lbm_residual": ([P, I, ctypes.POINTER(ctypes.c_double), P], I),
         "lbm_sync": ([P, P], I),
         "lbm_last_error": ([], ctypes.c_char_p),

@@ -861,6 +861,49 @@ int lbm_reduce_box(lbm_solver* s, const int origin[3], const int size[3], double
     return LBM_OK;
 }
 
+int lbm_reduce_boxes_noalloc(lbm_solver* const* solvers, const int* origins, const int* sizes,
+                             int count, double* out_dev, hipStream_t stream)
+{
+    if (!solvers || !out_dev) {
+        set_error("lbm_reduce_boxes_noalloc: solvers or out_dev is NULL");
+        return LBM_ERR_ARG;
+    }
+    if (count < 1 || count > LBM_NOALLOC_MAX_BOXES) {
+        char buf[160];
+        snprintf(buf, sizeof buf,
+                 "lbm_reduce_boxes_noalloc: count %d outside 1..%d (LBM_NOALLOC_MAX_BOXES, the "
+                 "size of the static partials array)",
+                 count, LBM_NOALLOC_MAX_BOXES);
+        set_error(buf);
+        return LBM_ERR_ARG;
+    }
+    Box boxes[LBM_NOALLOC_MAX_BOXES];
+    for (int m = 0; m < count; m++) {
+        const lbm_solver* s = solvers[m];
+        if (!s) {
+            set_error("lbm_reduce_boxes_noalloc: a solver is NULL");
+            return LBM_ERR_ARG;
+        }
+        if (s->p.device != solvers[0]->p.device || s->p.precision != solvers[0]->p.precision) {
+            set_error("lbm_reduce_boxes_noalloc: solvers differ in device or precision");
+            return LBM_ERR_ARG;
+        }
+        const int* o = origins ? origins + 3 * m : nullptr;
+        const int* z = sizes ? sizes + 3 * m : nullptr;
+        int rc = resolve_box(s, o, z, 0, &boxes[m]);
+        if (rc)
+            return rc;
+    }
+    int rc = set_device(solvers[0]);
+    if (rc)
+        return rc;
+    double* partials = nullptr;
+    LBM_HIP(noalloc_scratch(solvers[0]->p.device, &partials), "hipGetSymbolAddress(scratch)");
+    LBM_HIP(launch_reduce_boxes(solvers, boxes, count, partials, out_dev, stream),
+            "reduce kernel");
+    return LBM_OK;
+}
+
 int lbm_residual(lbm_solver* s, int layout, double* residual, hipStream_t stream)
 {
     if (!s || !residual || layout < 0 || layout > 1) {

@@ -134,6 +134,14 @@ hipError_t launch_residual_members(int precision, const ResidualMember* mem, int
 
 constexpr int kReduceBlocks = 1024;
 
+// lbm_reduce_boxes_noalloc: the device address of the static partials array
+// on `device` (which must be current), looked up on the first call per device.
+hipError_t noalloc_scratch(int device, double** ptr);
+// k_reduce_box for count (<= LBM_NOALLOC_MAX_BOXES) boxes of solvers of one
+// precision; partials: count * kReduceBlocks * 5 doubles, out: count * 5.
+hipError_t launch_reduce_boxes(const lbm_solver* const* s, const Box* b, int count,
+                               double* partials, double* out_dev, hipStream_t st);
+
 // Default launch options of the step kernels (kernel 0 = alpha, 1 = beta),
 // measured on MI355X: FP32 and FP64 gain from fewer resident waves (fewer
 // concurrent plane streams), 16-bit storage needs every wave; F64 beta from

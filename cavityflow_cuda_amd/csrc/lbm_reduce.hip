@@ -5,6 +5,9 @@
 #include "lbm_internal.hpp"
 #include "lbm_storage.hpp"
 
+#include <mutex>
+#include <unordered_map>
+
 namespace lbm {
 
 constexpr int kRedBlock = 256;
@@ -47,13 +50,16 @@ struct RedArgs {
     FastDiv div_bx, div_by;
 };
 
+// Block `block` of `nblocks` strides over the cells of the box; its sums go to
+// partials[block].
 template <class P>
-__global__ __launch_bounds__(kRedBlock) void k_reduce_box(const typename P::S* __restrict__ dd,
-                                                          RedArgs a, double* partials)
+__device__ __forceinline__ void reduce_box_block(const typename P::S* __restrict__ dd,
+                                                 const RedArgs& a, uint32_t block,
+                                                 uint32_t nblocks, double* partials)
 {
     double acc[kRedVals] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    for (uint32_t gid = blockIdx.x * kRedBlock + threadIdx.x; gid < a.cells;
-         gid += gridDim.x * kRedBlock) {
+    for (uint32_t gid = block * kRedBlock + threadIdx.x; gid < a.cells;
+         gid += nblocks * kRedBlock) {
         const uint32_t t = a.div_bx.div(gid);
         const uint32_t x = gid - t * a.div_bx.d;
         const uint32_t z = a.div_by.div(t);
@@ -70,7 +76,34 @@ __global__ __launch_bounds__(kRedBlock) void k_reduce_box(const typename P::S* _
         }
         acc[4] += 1.0;
     }
-    block_store(acc, partials, blockIdx.x);
+    block_store(acc, partials, block);
+}
+
+template <class P>
+__global__ __launch_bounds__(kRedBlock) void k_reduce_box(const typename P::S* __restrict__ dd,
+                                                          RedArgs a, double* partials)
+{
+    reduce_box_block<P>(dd, a, blockIdx.x, gridDim.x, partials);
+}
+
+// The box sums of several solvers in one launch: grid.y = member, and member
+// m's blocks do exactly what k_reduce_box's do for that box alone, into their
+// own slice of partials, so each member's sums carry lbm_reduce_box's bits.
+struct ReduceBoxMember {
+    const void* dd;
+    RedArgs a;
+};
+struct ReduceBoxMembers {
+    ReduceBoxMember m[LBM_NOALLOC_MAX_BOXES];
+};
+
+template <class P>
+__global__ __launch_bounds__(kRedBlock) void k_reduce_boxes(ReduceBoxMembers mem,
+                                                            double* partials)
+{
+    const ReduceBoxMember& m = mem.m[blockIdx.y];
+    reduce_box_block<P>((const typename P::S*)m.dd, m.a, blockIdx.x, gridDim.x,
+                        partials + (size_t)blockIdx.y * gridDim.x * kRedVals);
 }
 
 // Residual over FLUID cells. layout 0: slot l holds direction l (after beta);
@@ -191,9 +224,7 @@ __global__ __launch_bounds__(64) void k_finalize(const double* partials, int nbl
     }
 }
 
-template <class P>
-static hipError_t reduce_box_t(const lbm_solver* s, const Box& b, double* partials,
-                               double* out, hipStream_t st)
+static RedArgs red_args(const lbm_solver* s, const Box& b)
 {
     RedArgs a;
     a.stride = s->stride;
@@ -205,6 +236,14 @@ static hipError_t reduce_box_t(const lbm_solver* s, const Box& b, double* partia
     a.cells = b.cells();
     a.div_bx = FastDiv((uint32_t)b.bx);
     a.div_by = FastDiv((uint32_t)b.by);
+    return a;
+}
+
+template <class P>
+static hipError_t reduce_box_t(const lbm_solver* s, const Box& b, double* partials,
+                               double* out, hipStream_t st)
+{
+    const RedArgs a = red_args(s, b);
     hipLaunchKernelGGL(k_reduce_box<P>, dim3(kReduceBlocks), dim3(kRedBlock), 0, st,
                        (const typename P::S*)s->dd, a, partials);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, st, partials, kReduceBlocks, out);
@@ -227,16 +266,7 @@ template <class P>
 static hipError_t moments_t(const lbm_solver* s, const Box& b, int layout, void* dst,
                             hipStream_t st)
 {
-    RedArgs a;
-    a.stride = s->stride;
-    a.nx = s->size_h[0];
-    a.nxy = s->size_h[0] * s->size_h[1];
-    a.ox = b.ox;
-    a.oy = b.oy;
-    a.oz = b.oz;
-    a.cells = b.cells();
-    a.div_bx = FastDiv((uint32_t)b.bx);
-    a.div_by = FastDiv((uint32_t)b.by);
+    const RedArgs a = red_args(s, b);
     hipLaunchKernelGGL(k_moments<P>, dim3((a.cells + kRedBlock - 1) / kRedBlock),
                        dim3(kRedBlock), 0, st, (const typename P::S*)s->dd, a, layout,
                        (typename P::C*)dst);
@@ -288,6 +318,68 @@ hipError_t launch_residual_members(int precision, const ResidualMember* mem, int
         return residual_members_t<StoreF16S>(mem, count, layout, partials, out_dev, st);
     case LBM_BF16S:
         return residual_members_t<StoreBF16S>(mem, count, layout, partials, out_dev, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// ------------------------------------------------ allocation-free sums ---
+
+// Block partials of lbm_reduce_boxes_noalloc: LBM_NOALLOC_MAX_BOXES slices of
+// kReduceBlocks * kRedVals doubles (640 KiB). Every GPU the library runs on
+// holds its own copy until the process ends. In host code `scratch` names a
+// host-side stand-in, not a device address, so it is never passed to a kernel
+// or a copy: each device's address comes from hipGetSymbolAddress, looked up
+// once per device with that device current (noalloc_scratch). Every block
+// overwrites its whole slice, so nothing has to start at zero; two calls that
+// overlap on one GPU share the array and race.
+static __device__ __align__(256) double scratch[LBM_NOALLOC_MAX_BOXES * kReduceBlocks * kRedVals];
+
+static std::mutex g_scratch_mutex;
+static std::unordered_map<int, double*> g_scratch_ptr;
+
+hipError_t noalloc_scratch(int device, double** ptr)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    auto it = g_scratch_ptr.find(device);
+    if (it != g_scratch_ptr.end()) {
+        *ptr = it->second;
+        return hipSuccess;
+    }
+    // the caller has made `device` current
+    double* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess)
+        return e;
+    g_scratch_ptr.emplace(device, p);
+    *ptr = p;
+    return hipSuccess;
+}
+
+template <class P>
+static hipError_t reduce_boxes_t(const lbm_solver* const* s, const Box* b, int count,
+                                 double* partials, double* out, hipStream_t st)
+{
+    ReduceBoxMembers mem;
+    for (int m = 0; m < count; m++) {
+        mem.m[m].dd = s[m]->dd;
+        mem.m[m].a = red_args(s[m], b[m]);
+    }
+    hipLaunchKernelGGL(k_reduce_boxes<P>, dim3(kReduceBlocks, count), dim3(kRedBlock), 0, st,
+                       mem, partials);
+    hipLaunchKernelGGL(k_finalize, dim3(count), dim3(64), 0, st, partials, kReduceBlocks, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_boxes(const lbm_solver* const* s, const Box* b, int count,
+                               double* partials, double* out_dev, hipStream_t st)
+{
+    if (count < 1 || count > LBM_NOALLOC_MAX_BOXES)
+        return hipErrorInvalidValue;
+    switch (s[0]->p.precision) {
+    case LBM_F32: return reduce_boxes_t<StoreF32>(s, b, count, partials, out_dev, st);
+    case LBM_F64: return reduce_boxes_t<StoreF64>(s, b, count, partials, out_dev, st);
+    case LBM_F16S: return reduce_boxes_t<StoreF16S>(s, b, count, partials, out_dev, st);
+    case LBM_BF16S: return reduce_boxes_t<StoreBF16S>(s, b, count, partials, out_dev, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -315,6 +315,23 @@ int lbm_device_free(int device, void* ptr);
 int lbm_reduce_box(lbm_solver* solver, const int origin[3], const int size[3], double out[5],
                    hipStream_t stream);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): lbm_reduce_box of `count` solvers at once, allocating
+ * nothing. Box m is origins[3m..3m+2] / sizes[3m..3m+2] (NULL origins or
+ * sizes: every solver's whole halo-inclusive domain); the five sums of box m
+ * go to out_dev[5m..5m+4], caller-owned device memory, bit-identical to what
+ * lbm_reduce_box returns for that solver and box. The block partials live in
+ * a static device array of the library sized for LBM_NOALLOC_MAX_BOXES boxes
+ * (640 KiB per GPU, held until the process ends); count outside
+ * 1..LBM_NOALLOC_MAX_BOXES is rejected with LBM_ERR_ARG. All solvers must
+ * share one device and one precision. Asynchronous on `stream` (kernel
+ * launches only: no allocation, copy or synchronisation). Calls that overlap
+ * on one GPU share the partials array and race: order them on one stream. */
+#define LBM_NOALLOC_MAX_BOXES 16
+int lbm_reduce_boxes_noalloc(lbm_solver* const* solvers, const int* origins, const int* sizes,
+                             int count, double* out_dev, hipStream_t stream);
+
 /* Steady-state residual ||u - u_prev||_2 / ||u||_2 over FLUID cells, in FP64,
  * where u is the momentum computed from the populations (`layout` 0: after
  * beta, natural slots; 1: after alpha, swapped slots) and u_prev the value
