@@ -8,7 +8,8 @@ after every half-step. The unpack targets a shadow subdomain of the same shape
 so the cavity walls stay physical.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 512]
-                    [--precision f32|f64|f16s|bf16s] [--no-halo] [--cpu-sample-steps 2]
+                    [--precision f32|f64|f16s|bf16s|f16c] [--no-halo]
+                    [--cpu-sample-steps 2] [--workload step|precision|batched|serving|ceiling]
 
 Prints one JSON line. `value` is MLUPS = interior cells * K / t / 1e6 over the
 K timed steps (populations resident in HBM, no host transfers in the timed
@@ -36,7 +37,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "MLUPS & %HBM roofline; served cavity solves/s + p99 latency vs plain"
-BYTES_PER_POP = {"f32": 4, "f64": 8, "f16s": 2, "bf16s": 2}
+BYTES_PER_POP = {"f32": 4, "f64": 8, "f16s": 2, "bf16s": 2, "f16c": 2}
 
 
 class Hip:
@@ -119,6 +120,24 @@ def copy_bandwidth(hip, lib_, device, stream, gib=4.0, reps=10):
     return 2 * nbytes * reps / (ms * 1e-3) / 1e9
 
 
+def host_cpu():
+    """CPU model, logical CPUs of the host and CPUs this process may run on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return dict(cpu_model=model, nproc=os.cpu_count(), usable_cpus=usable)
+
+
 def cpu_baseline(size, params, sample_steps, threads_all):
     """Reference CPU kernels (oracle/_ref) on the same cavity, `sample_steps`
     whole-domain steps; returns (dict, dict-for-all-cores-oracle)."""
@@ -142,8 +161,9 @@ def cpu_baseline(size, params, sample_steps, threads_all):
     for n in range(sample_steps):
         s.beta() if n & 1 else s.alpha()
     dt = time.perf_counter() - t0
+    host = host_cpu()
     out["single"] = dict(value=round(cells * sample_steps / dt / 1e6, 3), unit="MLUPS",
-                         cores=threads, kind=kind,
+                         cores=threads, kind=kind, **host,
                          sample=f"{sample_steps} whole-domain steps (alpha+beta) of the same "
                                 f"{size[0]}x{size[1]}x{size[2]} FP32 cavity, "
                                 f"{'reference CLbmAlphaCPU/CLbmBetaCPU' if kind == 'reference' else 'oracle C restatement'}"
@@ -159,7 +179,7 @@ def cpu_baseline(size, params, sample_steps, threads_all):
             o.beta() if n & 1 else o.alpha()
         dt = time.perf_counter() - t0
         out["all"] = dict(value=round(cells * sample_steps / dt / 1e6, 3), unit="MLUPS",
-                          cores=threads_all, kind="port",
+                          cores=threads_all, kind="port", **host,
                           sample=f"{sample_steps} steps, oracle C restatement, OpenMP over z, "
                                  f"{dt:.2f} s")
     return out
@@ -390,84 +410,19 @@ def serving_workload(args, cf, L, hip, stream):
     print(json.dumps(line), flush=True)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--size", type=int, default=512)
-    ap.add_argument("--reynolds", type=float, default=1000.0)
-    ap.add_argument("--u-lid", type=float, default=0.1)
-    ap.add_argument("--precision", default="f32", choices=sorted(BYTES_PER_POP))
-    ap.add_argument("--no-halo", action="store_true")
-    ap.add_argument("--unpack", choices=("auto", "pair", "separate"), default="auto",
-                    help="x-face unpack: both faces in one call (pair) or one call each; "
-                         "auto = pair for 16-bit storage, separate otherwise")
-    ap.add_argument("--host-exchange", action="store_true",
-                    help="step: faces pass through pinned host buffers with the reference's "
-                         "calls (getDensityDistributions / setDensityDistributions), "
-                         "the PCIe-inclusive rate")
-    ap.add_argument("--no-fused-pack", action="store_true",
-                    help="pack the x faces with separate kernels instead of inside the steps")
-    ap.add_argument("--cpu-sample-steps", type=int, default=2)
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--skip-copy", action="store_true")
-    ap.add_argument("--launch-alpha", type=int, default=None,
-                    help="LBM_LAUNCH_* bits for the alpha kernel (default: library default)")
-    ap.add_argument("--launch-beta", type=int, default=None)
-    ap.add_argument("--workload", default="step",
-                    choices=("step", "precision", "batched", "serving"))
-    ap.add_argument("--seed", type=int, default=20261202)
-    ap.add_argument("--cavities", type=int, default=64, help="batched: number of cavities")
-    ap.add_argument("--no-graph", action="store_true",
-                    help="batched / serving: issue the batch launches directly instead of "
-                         "replaying captured graphs")
-    ap.add_argument("--trial-steps", type=int, default=8,
-                    help="batched / serving: time steps per autotuner trial")
-    ap.add_argument("--requests", type=int, default=200, help="serving: number of requests")
-    ap.add_argument("--max-steps", type=int, default=2000,
-                    help="serving: time-step cap per request")
-    ap.add_argument("--load", type=float, default=0.8,
-                    help="serving: arrival rate as a fraction of measured capacity "
-                         "(0: all requests arrive at once)")
-    ap.add_argument("--hbm-budget", type=float, default=200.0, help="serving: GB")
-    ap.add_argument("--policy", default="hrrn", choices=("srpt", "srpt_oldest", "hrrn", "all"),
-                    help="serving: which resident solves each round advances")
-    args = ap.parse_args()
-
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: the multi-GPU path (RCCL face exchange "
-              "between subdomains) is not part of this build; run with --gpus 1.",
-              file=sys.stderr)
-        sys.exit(2)
-
-    import cavityflow_cuda_amd as cf
-    from cavityflow_cuda_amd import _lib
+def step_run(args, cf, L, hip, stream, precision, steps, warmup):
+    """The step workload at one storage precision: warm-up, then `steps`
+    timed steps with HIP events around every step kernel on the launch
+    stream; returns the rates and per-kernel means."""
     from cavityflow_cuda_amd._lib import check
-
-    hip = Hip()
-    L = _lib.lib()
     device = 0
-    stream = hip.stream()
-    if args.workload == "precision":
-        precision_workload(args, cf, L, hip, stream)
-        return
-    if args.workload == "batched":
-        if args.steps == ap.get_default("steps"):
-            args.steps = 5000
-        batched_workload(args, cf, L, hip, stream)
-        return
-    if args.workload == "serving":
-        serving_workload(args, cf, L, hip, stream)
-        return
     size = (args.size,) * 3
-    params = cf.cavity_parameters(size, args.u_lid, args.reynolds, args.precision)
+    params = cf.cavity_parameters(size, args.u_lid, args.reynolds, precision)
 
-    print(f"[bench] cavity {size} {args.precision} Re={args.reynolds} lid={args.u_lid} "
+    print(f"[bench] cavity {size} {precision} Re={args.reynolds} lid={args.u_lid} "
           f"tau={params['tau']:.6f}", file=sys.stderr, flush=True)
     c = cf.CavityController(size, u_lid=args.u_lid, reynolds=args.reynolds,
-                            precision=args.precision, store_outputs=False, device=device,
+                            precision=precision, store_outputs=False, device=device,
                             params=params, stream=stream.value)
     solver = next(iter(c.solvers.values()))
     for k, opt in ((0, args.launch_alpha), (1, args.launch_beta)):
@@ -478,18 +433,18 @@ def main():
     bufs = []
     if halo:
         # shadow subdomain with the same halo planes: unpack target
-        shadow = cf.LbmSolverHIP(size, (cf.GHOST_LAYER,) * 6, args.precision,
+        shadow = cf.LbmSolverHIP(size, (cf.GHOST_LAYER,) * 6, precision,
                                  store_densities=False, store_velocities=False, device=device)
         for d in range(6):
             p = ctypes.c_void_p()
-            elem = 8 if args.precision == "f64" else 4
+            elem = 8 if precision == "f64" else 4
             check(L.lbm_device_alloc(device, solver.face_elements(d) * elem, ctypes.byref(p)))
             bufs.append(p)
 
     host_ex = halo and args.host_exchange
     hbufs = []
     if host_ex:
-        elem = 8 if args.precision == "f64" else 4
+        elem = 8 if precision == "f64" else 4
         hbufs = [hip.host_alloc(solver.face_elements(d) * elem) for d in range(6)]
     fused = halo and not args.no_fused_pack and not host_ex
     if fused:
@@ -506,7 +461,7 @@ def main():
     # halo-plane unpack dirties, which pays for 16-bit storage; with FP32 the
     # following beta absorbs the deferred write-back and loses what it saves.
     separate_unpack = (args.unpack == "separate" or
-                       (args.unpack == "auto" and args.precision in ("f32", "f64")))
+                       (args.unpack == "auto" and precision in ("f32", "f64")))
 
     def box3(box):
         return (ctypes.c_int * 3)(*box[0]), (ctypes.c_int * 3)(*box[1])
@@ -561,47 +516,38 @@ def main():
             halo_exchange(phase)
         counter[0] += 1
 
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         one_step()
     hip.sync(stream)
 
     # timed region: K steps, HIP events around every step kernel
-    nev = args.steps
+    nev = steps
     ev = [(hip.event(), hip.event()) for _ in range(nev)]
     t_start, t_end = hip.event(), hip.event()
     hip.sync(stream)
     hip.record(t_start, stream)
     w0 = time.perf_counter()
-    for i in range(args.steps):
+    for i in range(steps):
         one_step(ev[i])
     hip.record(t_end, stream)
     hip.sync(stream)
     wall = time.perf_counter() - w0
     total_ms = hip.elapsed_ms(t_start, t_end)
     durs = np.array([hip.elapsed_ms(a, b) for a, b in ev])
-    first_phase = args.warmup & 1
+    first_phase = warmup & 1
     alpha_ms = durs[(np.arange(nev) + first_phase) % 2 == 0]
     beta_ms = durs[(np.arange(nev) + first_phase) % 2 == 1]
 
     cells = int(np.prod(size))
-    mlups = cells * args.steps / (total_ms * 1e-3) / 1e6
-    bpp = BYTES_PER_POP[args.precision]
+    mlups = cells * steps / (total_ms * 1e-3) / 1e6
+    bpp = BYTES_PER_POP[precision]
     alg_bytes = 2 * 19 * bpp * cells
     kern = {"alpha": float(alpha_ms.mean()), "beta": float(beta_ms.mean())}
     dom = max(kern, key=kern.get)
     achieved = alg_bytes / (kern[dom] * 1e-3) / 1e9
-    traffic = measured_traffic(f"k_{dom}", args.precision, args.size)
+    traffic = measured_traffic(f"k_{dom}", precision, args.size)
     step_ms_total = float(durs.sum())
     halo_ms = total_ms - step_ms_total
-
-    copy_gbs = None
-    if not args.skip_copy:
-        copy_gbs = round(copy_bandwidth(hip, L, device, stream), 1)
-
-    cpu = {}
-    if not args.skip_cpu and args.cpu_sample_steps > 0:
-        print("[bench] CPU baseline sample", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(size, params, args.cpu_sample_steps, args.cpu_threads)
 
     c.close()
     if shadow is not None:
@@ -610,6 +556,120 @@ def main():
         L.lbm_device_free(device, p)
     for p in hbufs:
         hip.host_free(p)
+
+    return dict(size=size, params=params, mlups=mlups, total_ms=total_ms, kern=kern, dom=dom,
+                achieved=achieved, traffic=traffic, alg_bytes=alg_bytes, halo=halo, fused=fused,
+                host_ex=host_ex, separate_unpack=separate_unpack, launch_opts=launch_opts,
+                wall=wall, halo_ms=halo_ms, step_ms_total=step_ms_total, steps=steps)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--reynolds", type=float, default=1000.0)
+    ap.add_argument("--u-lid", type=float, default=0.1)
+    ap.add_argument("--precision", default="f32", choices=sorted(BYTES_PER_POP))
+    ap.add_argument("--no-halo", action="store_true")
+    ap.add_argument("--unpack", choices=("auto", "pair", "separate"), default="auto",
+                    help="x-face unpack: both faces in one call (pair) or one call each; "
+                         "auto = pair for 16-bit storage, separate otherwise")
+    ap.add_argument("--host-exchange", action="store_true",
+                    help="step: faces pass through pinned host buffers with the reference's "
+                         "calls (getDensityDistributions / setDensityDistributions), "
+                         "the PCIe-inclusive rate")
+    ap.add_argument("--no-fused-pack", action="store_true",
+                    help="pack the x faces with separate kernels instead of inside the steps")
+    ap.add_argument("--cpu-sample-steps", type=int, default=2)
+    # the GPU host gives one GPU's jobs a share of 16 CPUs (its OMP_NUM_THREADS)
+    ap.add_argument("--cpu-threads", type=int,
+                    default=int(os.environ.get("OMP_NUM_THREADS", 0)) or
+                    min(16, os.cpu_count() or 1))
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-copy", action="store_true")
+    ap.add_argument("--no-shifted", action="store_true",
+                    help="step, f32: skip the F16S / F16C runs reported in `extra`")
+    ap.add_argument("--launch-alpha", type=int, default=None,
+                    help="LBM_LAUNCH_* bits for the alpha kernel (default: library default)")
+    ap.add_argument("--launch-beta", type=int, default=None)
+    ap.add_argument("--workload", default="step",
+                    choices=("step", "precision", "batched", "serving"))
+    ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--cavities", type=int, default=64, help="batched: number of cavities")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="batched / serving: issue the batch launches directly instead of "
+                         "replaying captured graphs")
+    ap.add_argument("--trial-steps", type=int, default=8,
+                    help="batched / serving: time steps per autotuner trial")
+    ap.add_argument("--requests", type=int, default=200, help="serving: number of requests")
+    ap.add_argument("--max-steps", type=int, default=2000,
+                    help="serving: time-step cap per request")
+    ap.add_argument("--load", type=float, default=0.8,
+                    help="serving: arrival rate as a fraction of measured capacity "
+                         "(0: all requests arrive at once)")
+    ap.add_argument("--hbm-budget", type=float, default=200.0, help="serving: GB")
+    ap.add_argument("--policy", default="hrrn", choices=("srpt", "srpt_oldest", "hrrn", "all"),
+                    help="serving: which resident solves each round advances")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the multi-GPU path (RCCL face exchange "
+              "between subdomains) is not part of this build; run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import cavityflow_cuda_amd as cf
+    from cavityflow_cuda_amd import _lib
+    from cavityflow_cuda_amd._lib import check
+
+    hip = Hip()
+    L = _lib.lib()
+    device = 0
+    stream = hip.stream()
+    if args.workload == "precision":
+        precision_workload(args, cf, L, hip, stream)
+        return
+    if args.workload == "batched":
+        if args.steps == ap.get_default("steps"):
+            args.steps = 5000
+        batched_workload(args, cf, L, hip, stream)
+        return
+    if args.workload == "serving":
+        serving_workload(args, cf, L, hip, stream)
+        return
+    run = step_run(args, cf, L, hip, stream, args.precision, args.steps, args.warmup)
+    size, params, mlups, total_ms = run["size"], run["params"], run["mlups"], run["total_ms"]
+    kern, dom, achieved, traffic = run["kern"], run["dom"], run["achieved"], run["traffic"]
+    alg_bytes, halo, fused, host_ex = run["alg_bytes"], run["halo"], run["fused"], run["host_ex"]
+    separate_unpack, launch_opts = run["separate_unpack"], run["launch_opts"]
+    wall, halo_ms, step_ms_total = run["wall"], run["halo_ms"], run["step_ms_total"]
+
+    # the shifted 16-bit formats on the same workload, for the ratio to FP32
+    # (fewer steps: the rate is steady after a few hundred)
+    shifted = {}
+    if args.precision == "f32" and not args.no_shifted:
+        for prec in ("f16s", "f16c"):
+            r = step_run(args, cf, L, hip, stream, prec, min(args.steps, 1000), args.warmup)
+            shifted[prec] = dict(mlups=round(r["mlups"], 1),
+                                 ratio_to_f32=round(r["mlups"] / mlups, 3),
+                                 steps=r["steps"],
+                                 kernel_ms={k: round(v, 4) for k, v in r["kern"].items()},
+                                 roofline_frac={k: round(r["alg_bytes"] / (v * 1e-3) / 1e9
+                                                         / HBM_PEAK_GBS, 4)
+                                                for k, v in r["kern"].items()},
+                                 launch_options=r["launch_opts"])
+            print(f"[bench] {prec}: {shifted[prec]}", file=sys.stderr, flush=True)
+
+    copy_gbs = None
+    if not args.skip_copy:
+        copy_gbs = round(copy_bandwidth(hip, L, 0, stream), 1)
+
+    cpu = {}
+    if not args.skip_cpu and args.cpu_sample_steps > 0:
+        print("[bench] CPU baseline sample", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(size, params, args.cpu_sample_steps, args.cpu_threads)
 
     t_elem = 8 if args.precision == "f64" else 4
     line = {
@@ -651,6 +711,9 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "traffic_source": ("rocprofv3 FETCH_SIZE + WRITE_SIZE per launch of this workload, "
+                               "from the committed profile profiles/step_f32_512_hbm_traffic.md "
+                               "(not measured in this run)") if traffic else None,
             # measured HBM bytes (rocprof, per launch) over this run's launch time
             "traffic_frac": (round(traffic / (kern[dom] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
                              if traffic else None),
@@ -666,6 +729,7 @@ def main():
             "device_copy_gbs": copy_gbs,
             "reference_glups": round(mlups / 1e3, 4),
             "reference_bandwidth_gbs": round(mlups / 1e3 * (19 * 2 * t_elem + 4), 1),
+            "shifted_storage": shifted or None,
         },
     }
     print(json.dumps(line), flush=True)

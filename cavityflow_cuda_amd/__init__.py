@@ -13,7 +13,7 @@ saleemayman/cavityflow_cuda behind that project's CLbmSolver operator surface:
 There is no CPU fallback: using the package without the built HIP library
 raises.
 """
-from ._lib import (BACK, BF16S, BOTTOM, F16S, F32, F64, FLUID, FRONT, GHOST_LAYER, LEFT,
+from ._lib import (BACK, BF16S, BOTTOM, F16C, F16S, F32, F64, FLUID, FRONT, GHOST_LAYER, LEFT,
                    OBSTACLE, RIGHT, TOP, VELOCITY_INJECTION, LbmError)
 from .solver import LbmSolverHIP, compute_parameters
 from .controller import CavityController, cavity_parameters
@@ -21,5 +21,5 @@ from .controller import CavityController, cavity_parameters
 __all__ = [
     "LbmSolverHIP", "CavityController", "cavity_parameters", "compute_parameters", "LbmError",
     "OBSTACLE", "FLUID", "VELOCITY_INJECTION", "GHOST_LAYER", "LEFT", "RIGHT", "BOTTOM", "TOP",
-    "BACK", "FRONT", "F32", "F64", "F16S", "BF16S",
+    "BACK", "FRONT", "F32", "F64", "F16S", "BF16S", "F16C",
 ]

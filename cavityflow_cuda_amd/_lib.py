@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "libcavityflow_hip.so")
 # enum lbm_flag / lbm_direction / lbm_precision / lbm_status
 OBSTACLE, FLUID, VELOCITY_INJECTION, GHOST_LAYER = 1, 2, 4, 8
 LEFT, RIGHT, BOTTOM, TOP, BACK, FRONT = range(6)
-F32, F64, F16S, BF16S = 0, 1, 2, 3
+F32, F64, F16S, BF16S, F16C = 0, 1, 2, 3, 4
 LAUNCH_NONTEMPORAL, LAUNCH_XCD_REMAP, LAUNCH_WAVES4, LAUNCH_WAVES3, LAUNCH_WAVES2 = 1, 2, 4, 8, 16
 LAUNCH_ADDR64 = 32
 LAUNCH_WIDE = 64

@@ -3,7 +3,7 @@
 // the C++ solver surface CLbmSolverHIP<T>, on one GPU.
 //
 //   cavity_driver --size NX NY NZ [--split PX PY PZ] [--u-lid U] [--re RE]
-//                 [--loops N] [--precision f32|f64|f16s|bf16s] [--dump FILE]
+//                 [--loops N] [--precision f32|f64|f16s|bf16s|f16c] [--dump FILE]
 //   cavity_driver --config FILE.xml [--precision ...] [--loops N] [--dump FILE]
 //                 [--vis-format netcdf|netcdf-global|vtk]
 //   cavity_driver --config FILE.xml --print-config
@@ -78,6 +78,7 @@ int precision_code(const std::string& p)
     if (p == "f64") return LBM_F64;
     if (p == "f16s") return LBM_F16S;
     if (p == "bf16s") return LBM_BF16S;
+    if (p == "f16c") return LBM_F16C;
     throw lbm::Error(LBM_ERR_ARG, "unknown precision " + p);
 }
 

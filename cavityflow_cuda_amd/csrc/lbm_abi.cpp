@@ -52,6 +52,7 @@ int element_bytes(int precision)
     case LBM_F64: return 8;
     case LBM_F16S: return 2;
     case LBM_BF16S: return 2;
+    case LBM_F16C: return 2;
     default: return 0;
     }
 }
@@ -322,7 +323,7 @@ int lbm_device_bytes(const lbm_params* params, int pair_buffer, uint64_t* bytes)
     uint64_t b = (uint64_t)kQ * stride * eb + stride;                  // populations, flags
     b += params->store_densities ? stride * ob : 0;
     b += params->store_velocities ? 3 * stride * ob : 0;
-    b += (uint64_t)(kReduceBlocks + 1) * 5 * sizeof(double);          // reduction partials
+    b += kPartialsBytes;                                               // reduction partials
     b += 3 * stride * ob;                                              // residual snapshot
     b += pair_buffer ? (uint64_t)kQ * stride * eb : 0;                 // second populations
     *bytes = b;
@@ -382,9 +383,10 @@ int lbm_create(const lbm_params* params, lbm_solver** out)
     if (p.store_velocities &&
         hipMalloc(&s->u, (size_t)3 * s->stride * s->out_bytes) != hipSuccess)
         return fail(LBM_ERR_NOMEM, "cannot allocate velocities");
-    if (hipMalloc((void**)&s->partials, (size_t)(kReduceBlocks + 1) * 5 * sizeof(double)) !=
-        hipSuccess)
+    if (hipMalloc((void**)&s->partials, kPartialsBytes) != hipSuccess)
         return fail(LBM_ERR_NOMEM, "cannot allocate reduction buffer");
+    // the overflow word after the partials; the init kernel clears it
+    s->status = (uint32_t*)(s->partials + (kReduceBlocks + 1) * 5);
     hipError_t e = launch_init(s, nullptr, 1);
     if (e == hipSuccess)
         e = hipStreamSynchronize(nullptr);
@@ -928,12 +930,8 @@ int lbm_residual(lbm_solver* s, int layout, double* residual, hipStream_t stream
     LBM_HIP(hipMemcpyAsync(host, res, 5 * sizeof(double), hipMemcpyDeviceToHost, stream),
             "D2H copy");
     LBM_HIP(hipStreamSynchronize(stream), "hipStreamSynchronize");
-    if (!s->residual_valid) {
-        *residual = std::numeric_limits<double>::infinity();
-        s->residual_valid = 1;
-    } else {
-        *residual = residual_ratio(host[0], host[1]);
-    }
+    *residual = first_residual(s->residual_valid, host[0], host[1]);
+    s->residual_valid = 1;
     return LBM_OK;
 }
 
@@ -990,6 +988,8 @@ int state_copy(lbm_solver* s, void* host, uint64_t bytes, bool save)
         LBM_HIP(hipMemcpy2D(d, pitch, h, plane, plane, kQ, hipMemcpyHostToDevice), "H2D copy");
         LBM_HIP(hipMemcpy(s->flags, h + kQ * plane, (size_t)s->n, hipMemcpyHostToDevice),
                 "H2D copy");
+        // a saved state was stored by the same formats, hence in range
+        LBM_HIP(hipMemset(s->status, 0, sizeof(uint32_t)), "hipMemset");
         s->residual_valid = 0;
     }
     return LBM_OK;

@@ -35,6 +35,7 @@ struct MemberDesc {
     uint32_t tile_begin;
     int32_t store_rho, store_u;
     double inv_tau, lid, g0, g1, g2;
+    uint32_t* status;
 };
 
 template <class C>
@@ -58,6 +59,7 @@ __device__ __forceinline__ StepArgs<C> member_args(const MemberDesc& d, int phas
     a.store_u = d.store_u;
     a.xpack_on = 0;
     a.xpack[0] = a.xpack[1] = nullptr;
+    a.status = d.status;
 #pragma unroll
     for (int l = 0; l < kQ; l++)
         a.plane[l] = d.plane[phase][l];
@@ -307,6 +309,7 @@ int prepare_pairs(lbm_batch* b, int pairs)
         d.g0 = h.g0;
         d.g1 = h.g1;
         d.g2 = h.g2;
+        d.status = s->status;
         map.insert(map.end(), (size_t)d.tiles_x * d.tiles_y * d.chunks, (int32_t)m);
     }
     if (b->pair_member)
@@ -394,6 +397,7 @@ hipError_t launch_pair(const lbm_batch* b, int parity, hipStream_t st)
     case LBM_F32: return launch_pair_p<StoreF32>(b, parity, st);
     case LBM_F16S: return launch_pair_p<StoreF16S>(b, parity, st);
     case LBM_BF16S: return launch_pair_p<StoreBF16S>(b, parity, st);
+    case LBM_F16C: return launch_pair_p<StoreF16C>(b, parity, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -443,6 +447,7 @@ hipError_t launch_tiles(const lbm_batch* b, uint32_t lo, uint32_t hi, int phase,
     case LBM_F64: return launch_tiles_p<StoreF64>(b, lo, hi, phase, st);
     case LBM_F16S: return launch_tiles_p<StoreF16S>(b, lo, hi, phase, st);
     case LBM_BF16S: return launch_tiles_p<StoreBF16S>(b, lo, hi, phase, st);
+    case LBM_F16C: return launch_tiles_p<StoreF16C>(b, lo, hi, phase, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -547,6 +552,7 @@ int lbm_batch_create(lbm_solver* const* members, int count, lbm_batch** out)
         d.g0 = s->p.gravitation[0];
         d.g1 = s->p.gravitation[1];
         d.g2 = s->p.gravitation[2];
+        d.status = s->status;
     }
     lbm_batch_tuning t;
     t.block = 256;
@@ -860,7 +866,7 @@ int lbm_batch_residual(lbm_batch* b, int layout, double* residuals, hipStream_t 
                 s->residual_valid = 0;
             }
             mem[i] = ResidualMember{s->dd, s->flags, s->residual_prev, s->stride,
-                                    (uint32_t)s->n};
+                                    (uint32_t)s->n, s->status};
         }
         if (hipMalloc((void**)&b->res_mem, mem.size() * sizeof(ResidualMember)) != hipSuccess ||
             hipMalloc((void**)&b->res_part,
@@ -883,12 +889,8 @@ int lbm_batch_residual(lbm_batch* b, int layout, double* residuals, hipStream_t 
     // lbm_residual and cleared by lbm_reset
     for (int i = 0; i < count; i++) {
         lbm_solver* s = b->members[i];
-        if (!s->residual_valid) {
-            residuals[i] = std::numeric_limits<double>::infinity();
-            s->residual_valid = 1;
-        } else {
-            residuals[i] = residual_ratio(host[5 * i], host[5 * i + 1]);
-        }
+        residuals[i] = first_residual(s->residual_valid, host[5 * i], host[5 * i + 1]);
+        s->residual_valid = 1;
     }
     return LBM_OK;
 }

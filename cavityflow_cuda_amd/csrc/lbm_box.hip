@@ -18,6 +18,7 @@ struct BoxArgs {
     uint32_t cells;
     FastDiv div_bx, div_by;
     uint32_t mask;
+    uint32_t* status;   // raised by a population store out of the storage range
 };
 
 __device__ __forceinline__ uint32_t box_index(const BoxArgs& a, uint32_t gid)
@@ -59,7 +60,9 @@ __global__ __launch_bounds__(kCopyBlock) void k_set_dd(typename P::S* __restrict
     if (gid >= a.cells)
         return;
     const uint32_t id = box_index(a, gid);
-    const typename P::S v = P::store(src[(int64_t)l * a.cells + gid], l);
+    uint32_t ov = 0;
+    const typename P::S v = P::store_checked(src[(int64_t)l * a.cells + gid], l, ov);
+    raise_overflow<P>(a.status, ov);
     if (NT)
         __builtin_nontemporal_store(v, dd + (int64_t)l * a.stride + id);
     else
@@ -79,7 +82,7 @@ __global__ __launch_bounds__(kCopyBlock) void k_set_dd_xpair(typename P::S* __re
                                                              int64_t stride, int32_t nx,
                                                              int32_t x_lo, int32_t x_hi,
                                                              uint32_t rows, uint32_t mask_lo,
-                                                             uint32_t mask_hi)
+                                                             uint32_t mask_hi, uint32_t* status)
 {
     const int l = blockIdx.y;
     const bool on_lo = (mask_lo >> l) & 1u, on_hi = (mask_hi >> l) & 1u;
@@ -95,10 +98,12 @@ __global__ __launch_bounds__(kCopyBlock) void k_set_dd_xpair(typename P::S* __re
         else
             p[i] = v;
     };
+    uint32_t ov = 0;
     if (on_hi && r > 0)
-        put((int64_t)(r - 1) * nx + x_hi, P::store(hi[(int64_t)l * rows + r - 1], l));
+        put((int64_t)(r - 1) * nx + x_hi, P::store_checked(hi[(int64_t)l * rows + r - 1], l, ov));
     if (on_lo && r < rows)
-        put((int64_t)r * nx + x_lo, P::store(lo[(int64_t)l * rows + r], l));
+        put((int64_t)r * nx + x_lo, P::store_checked(lo[(int64_t)l * rows + r], l, ov));
+    raise_overflow<P>(status, ov);
 }
 
 // rho / u fields (compute type, no conversion)
@@ -156,6 +161,7 @@ static BoxArgs box_args(const lbm_solver* s, const Box& b, uint32_t mask)
     a.div_bx = FastDiv((uint32_t)b.bx);
     a.div_by = FastDiv((uint32_t)b.by);
     a.mask = mask;
+    a.status = s->status;
     return a;
 }
 
@@ -212,6 +218,7 @@ hipError_t launch_box_get(const lbm_solver* s, int field, const Box& b, void* ds
         case LBM_F64: return get_dd_t<StoreF64>(s, b, dst, st);
         case LBM_F16S: return get_dd_t<StoreF16S>(s, b, dst, st);
         case LBM_BF16S: return get_dd_t<StoreBF16S>(s, b, dst, st);
+        case LBM_F16C: return get_dd_t<StoreF16C>(s, b, dst, st);
         default: return hipErrorInvalidValue;
         }
     }
@@ -229,6 +236,7 @@ hipError_t launch_box_set(const lbm_solver* s, int field, const Box& b, const vo
         case LBM_F64: return set_dd_t<StoreF64>(s, b, src, mask, st);
         case LBM_F16S: return set_dd_t<StoreF16S>(s, b, src, mask, st);
         case LBM_BF16S: return set_dd_t<StoreBF16S>(s, b, src, mask, st);
+        case LBM_F16C: return set_dd_t<StoreF16C>(s, b, src, mask, st);
         default: return hipErrorInvalidValue;
         }
     }
@@ -248,12 +256,12 @@ static hipError_t set_dd_xpair_t(const lbm_solver* s, const void* lo, const void
         hipLaunchKernelGGL((k_set_dd_xpair<P, true>), grid, dim3(kCopyBlock), 0, st,
                            (typename P::S*)s->dd, (const typename P::C*)lo,
                            (const typename P::C*)hi, s->stride, s->size_h[0], x_lo, x_hi, rows,
-                           mask_lo, mask_hi);
+                           mask_lo, mask_hi, s->status);
     else
         hipLaunchKernelGGL((k_set_dd_xpair<P, false>), grid, dim3(kCopyBlock), 0, st,
                            (typename P::S*)s->dd, (const typename P::C*)lo,
                            (const typename P::C*)hi, s->stride, s->size_h[0], x_lo, x_hi, rows,
-                           mask_lo, mask_hi);
+                           mask_lo, mask_hi, s->status);
     return hipGetLastError();
 }
 
@@ -270,6 +278,8 @@ hipError_t launch_set_dd_xpair(const lbm_solver* s, const void* lo, const void* 
         return set_dd_xpair_t<StoreF16S>(s, lo, hi, x_lo, x_hi, mask_lo, mask_hi, nt, st);
     case LBM_BF16S:
         return set_dd_xpair_t<StoreBF16S>(s, lo, hi, x_lo, x_hi, mask_lo, mask_hi, nt, st);
+    case LBM_F16C:
+        return set_dd_xpair_t<StoreF16C>(s, lo, hi, x_lo, x_hi, mask_lo, mask_hi, nt, st);
     default: return hipErrorInvalidValue;
     }
 }

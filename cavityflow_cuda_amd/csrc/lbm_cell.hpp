@@ -274,9 +274,11 @@ __device__ __forceinline__ void alpha_cell(const StepArgs<typename P::C>& a, uin
         LBM_RELAX(18, eq_18(dd_param));
 #undef LBM_RELAX
         // direction l goes to slot opp(l) of this cell
+        uint32_t ov = 0;
 #pragma unroll
         for (int l = 0; l < kQ; l++)
-            acc.store(opp(l), 0, P::store(e[l], opp(l)));
+            acc.store(opp(l), 0, P::store_checked(e[l], opp(l), ov));
+        raise_overflow<P>(a.status, ov);
         if (pack) {
             C out[kQ];
 #pragma unroll
@@ -431,9 +433,11 @@ __device__ __forceinline__ void beta_cell(const StepArgs<typename P::C>& a, uint
 
     if (flag != kObstacle) {
         // direction l goes to slot l of neighbour id + c_l
+        uint32_t ov = 0;
 #pragma unroll
         for (int l = 0; l < kQ; l++)
-            acc.store(l, cx(l) + cy(l) * a.nx + cz(l) * a.nxy, P::store(d[l], l));
+            acc.store(l, cx(l) + cy(l) * a.nx + cz(l) * a.nxy, P::store_checked(d[l], l, ov));
+        raise_overflow<P>(a.status, ov);
         if (pack)
             xpack_beta<P>(a, row, pl, pr, d);
     }

@@ -65,6 +65,8 @@ struct StepArgs {
     FastDiv div_nx;
     int32_t ny;
     int32_t fcells;   // ny * nz
+    // set to 1 by a store the storage format cannot represent (F16C)
+    uint32_t* status;
 };
 
 }  // namespace lbm
@@ -89,6 +91,7 @@ struct lbm_solver {
     int launch_set[2]; // set by the caller: kept when the step pack changes
     void* xpack[2];    // fused x-face pack buffers (LEFT, RIGHT), NULL when off
     void* dd_alt;      // second population buffer of the out-of-place pair pass
+    uint32_t* status;  // device word: nonzero after a store out of the storage range
 };
 
 namespace lbm {
@@ -127,12 +130,16 @@ struct ResidualMember {
     void* prev;
     int64_t stride;
     uint32_t n;
+    const uint32_t* status;
 };
 hipError_t launch_residual_members(int precision, const ResidualMember* mem, int count,
                                    int layout, double* partials, double* out_dev,
                                    hipStream_t st);
 
 constexpr int kReduceBlocks = 1024;
+// Per-solver reduction buffer: kReduceBlocks * 5 partial sums, 5 results,
+// then the solver's overflow word (lbm_solver::status).
+constexpr size_t kPartialsBytes = (size_t)(kReduceBlocks + 1) * 5 * sizeof(double) + 64;
 
 // lbm_reduce_boxes_noalloc: the device address of the static partials array
 // on `device` (which must be current), looked up on the first call per device.
@@ -180,6 +187,17 @@ inline double residual_ratio(double num, double den)
     if (den > 0)
         return std::sqrt(num / den);
     return num > 0 ? std::numeric_limits<double>::infinity() : 0.0;
+}
+
+// The residual call's value: +inf on the first call after creation, reset or
+// load (no previous field to compare with) unless the state is already
+// non-finite, then residual_ratio.
+inline double first_residual(int valid, double num, double den)
+{
+    if (valid)
+        return residual_ratio(num, den);
+    const double r = residual_ratio(num, den);
+    return std::isnan(r) ? r : std::numeric_limits<double>::infinity();
 }
 
 }  // namespace lbm

@@ -54,6 +54,7 @@ struct PairDesc {
     int32_t tiles_x, tiles_y, chunks;
     uint32_t tile_begin;
     double inv_tau, lid, g0, g1, g2;
+    uint32_t* status;
 };
 
 constexpr int kOutside = 0x100;                       // window cell outside the array
@@ -155,6 +156,7 @@ __device__ __forceinline__ StepArgs<C> pair_args(const PairDesc& d)
     a.store_u = 0;
     a.xpack_on = 0;
     a.xpack[0] = a.xpack[1] = nullptr;
+    a.status = d.status;
     return a;
 }
 

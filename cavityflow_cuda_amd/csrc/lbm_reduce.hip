@@ -48,7 +48,15 @@ struct RedArgs {
     int32_t ox, oy, oz;
     uint32_t cells;
     FastDiv div_bx, div_by;
+    const uint32_t* status;   // the solver's overflow word
 };
+
+// A state with a population store out of its storage range (F16C) has no
+// meaningful sums: they become NaN, as a non-finite state's are.
+__device__ __forceinline__ bool poisoned(const uint32_t* status)
+{
+    return status && *status != 0u;
+}
 
 // Block `block` of `nblocks` strides over the cells of the box; its sums go to
 // partials[block].
@@ -76,6 +84,9 @@ __device__ __forceinline__ void reduce_box_block(const typename P::S* __restrict
         }
         acc[4] += 1.0;
     }
+    if (poisoned(a.status))
+        for (int k = 0; k < 4; k++)
+            acc[k] = __builtin_nan("");
     block_store(acc, partials, block);
 }
 
@@ -115,7 +126,7 @@ __device__ __forceinline__ void residual_block(const typename P::S* __restrict__
                                                typename P::C* __restrict__ prev,
                                                int64_t stride, uint32_t n, int layout,
                                                uint32_t block, uint32_t nblocks,
-                                               double* partials)
+                                               double* partials, const uint32_t* status)
 {
     using C = typename P::C;
     double acc[kRedVals] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -143,6 +154,8 @@ __device__ __forceinline__ void residual_block(const typename P::S* __restrict__
         prev[stride + id] = uy;
         prev[2 * stride + id] = uz;
     }
+    if (poisoned(status))
+        acc[0] = acc[1] = __builtin_nan("");
     block_store(acc, partials, block);
 }
 
@@ -151,9 +164,11 @@ __global__ __launch_bounds__(kRedBlock) void k_residual(const typename P::S* __r
                                                         const uint8_t* __restrict__ flags,
                                                         typename P::C* __restrict__ prev,
                                                         int64_t stride, uint32_t n, int layout,
-                                                        double* partials)
+                                                        double* partials,
+                                                        const uint32_t* status)
 {
-    residual_block<P>(dd, flags, prev, stride, n, layout, blockIdx.x, gridDim.x, partials);
+    residual_block<P>(dd, flags, prev, stride, n, layout, blockIdx.x, gridDim.x, partials,
+                      status);
 }
 
 // The residual of many solvers in one launch: grid.y = member, and member m's
@@ -167,7 +182,7 @@ __global__ __launch_bounds__(kRedBlock) void k_residual_members(const ResidualMe
     const ResidualMember& m = mem[blockIdx.y];
     residual_block<P>((const typename P::S*)m.dd, m.flags, (typename P::C*)m.prev, m.stride,
                       m.n, layout, blockIdx.x, gridDim.x,
-                      partials + (size_t)blockIdx.y * gridDim.x * kRedVals);
+                      partials + (size_t)blockIdx.y * gridDim.x * kRedVals, m.status);
 }
 
 // Density and momentum of every cell of a box, computed from the populations
@@ -198,6 +213,8 @@ __global__ __launch_bounds__(kRedBlock) void k_moments(const typename P::S* __re
     }
     C rho, ux, uy, uz;
     moments(d, rho, ux, uy, uz);
+    if (poisoned(a.status))
+        rho = ux = uy = uz = (C)__builtin_nan("");
     dst[gid] = rho;
     dst[(int64_t)a.cells + gid] = ux;
     dst[2 * (int64_t)a.cells + gid] = uy;
@@ -236,6 +253,7 @@ static RedArgs red_args(const lbm_solver* s, const Box& b)
     a.cells = b.cells();
     a.div_bx = FastDiv((uint32_t)b.bx);
     a.div_by = FastDiv((uint32_t)b.by);
+    a.status = s->status;
     return a;
 }
 
@@ -257,7 +275,7 @@ static hipError_t residual_t(const lbm_solver* s, int layout, double* partials, 
     hipLaunchKernelGGL(k_residual<P>, dim3(kReduceBlocks), dim3(kRedBlock), 0, st,
                        (const typename P::S*)s->dd, s->flags,
                        (typename P::C*)s->residual_prev, s->stride, (uint32_t)s->n, layout,
-                       partials);
+                       partials, s->status);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, st, partials, kReduceBlocks, out);
     return hipGetLastError();
 }
@@ -281,6 +299,7 @@ hipError_t launch_moments(const lbm_solver* s, const Box& b, int layout, void* d
     case LBM_F64: return moments_t<StoreF64>(s, b, layout, dst, st);
     case LBM_F16S: return moments_t<StoreF16S>(s, b, layout, dst, st);
     case LBM_BF16S: return moments_t<StoreBF16S>(s, b, layout, dst, st);
+    case LBM_F16C: return moments_t<StoreF16C>(s, b, layout, dst, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -293,6 +312,7 @@ hipError_t launch_reduce_box(const lbm_solver* s, const Box& b, double* partials
     case LBM_F64: return reduce_box_t<StoreF64>(s, b, partials, out_dev, st);
     case LBM_F16S: return reduce_box_t<StoreF16S>(s, b, partials, out_dev, st);
     case LBM_BF16S: return reduce_box_t<StoreBF16S>(s, b, partials, out_dev, st);
+    case LBM_F16C: return reduce_box_t<StoreF16C>(s, b, partials, out_dev, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -318,6 +338,8 @@ hipError_t launch_residual_members(int precision, const ResidualMember* mem, int
         return residual_members_t<StoreF16S>(mem, count, layout, partials, out_dev, st);
     case LBM_BF16S:
         return residual_members_t<StoreBF16S>(mem, count, layout, partials, out_dev, st);
+    case LBM_F16C:
+        return residual_members_t<StoreF16C>(mem, count, layout, partials, out_dev, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -380,6 +402,7 @@ hipError_t launch_reduce_boxes(const lbm_solver* const* s, const Box* b, int cou
     case LBM_F64: return reduce_boxes_t<StoreF64>(s, b, count, partials, out_dev, st);
     case LBM_F16S: return reduce_boxes_t<StoreF16S>(s, b, count, partials, out_dev, st);
     case LBM_BF16S: return reduce_boxes_t<StoreBF16S>(s, b, count, partials, out_dev, st);
+    case LBM_F16C: return reduce_boxes_t<StoreF16C>(s, b, count, partials, out_dev, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -392,6 +415,7 @@ hipError_t launch_residual(const lbm_solver* s, int layout, double* partials, do
     case LBM_F64: return residual_t<StoreF64>(s, layout, partials, out_dev, st);
     case LBM_F16S: return residual_t<StoreF16S>(s, layout, partials, out_dev, st);
     case LBM_BF16S: return residual_t<StoreBF16S>(s, layout, partials, out_dev, st);
+    case LBM_F16C: return residual_t<StoreF16C>(s, layout, partials, out_dev, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -59,6 +59,8 @@ __global__ __launch_bounds__(kBlock) void k_init(StepArgs<typename P::C> a, int 
         flag = bc[5];
     if (write_flags)
         const_cast<uint8_t*>(a.flags)[id] = (uint8_t)flag;
+    if (id == 0)   // the rest state is in range for every storage format
+        *a.status = 0u;
 
     S* dd = (S*)a.dd;
     const C ux = (C)0, uy = (C)0, uz = (C)0, rho = (C)1;
@@ -282,6 +284,7 @@ static StepArgs<typename P::C> make_args(const lbm_solver* s, const Box& b)
     a.div_nx = FastDiv((uint32_t)s->size_h[0]);
     a.ny = s->size_h[1];
     a.fcells = s->size_h[1] * s->size_h[2];
+    a.status = s->status;
     for (int l = 0; l < kQ; l++)
         a.plane[l] = (const char*)s->dd + (int64_t)l * s->stride * (int64_t)sizeof(typename P::S);
     return a;
@@ -422,6 +425,7 @@ static hipError_t beta_t(const lbm_solver* s, const Box& b, hipStream_t st)
     case LBM_F64: return fn<StoreF64>(__VA_ARGS__);            \
     case LBM_F16S: return fn<StoreF16S>(__VA_ARGS__);          \
     case LBM_BF16S: return fn<StoreBF16S>(__VA_ARGS__);        \
+    case LBM_F16C: return fn<StoreF16C>(__VA_ARGS__);          \
     default: return hipErrorInvalidValue;                      \
     }
 

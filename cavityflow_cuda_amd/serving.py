@@ -55,7 +55,7 @@ from .controller import cavity_parameters
 from .solver import LbmSolverHIP, _stream, device_bytes
 
 
-_ELEM_BYTES = {"f32": 4, "f64": 8, "f16s": 2, "bf16s": 2}
+_ELEM_BYTES = {"f32": 4, "f64": 8, "f16s": 2, "bf16s": 2, "f16c": 2}
 
 
 @dataclass

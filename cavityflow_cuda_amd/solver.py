@@ -22,7 +22,8 @@ import numpy as np
 from . import _lib
 from ._lib import LbmError, check, lib
 
-_PRECISIONS = {"f32": _lib.F32, "f64": _lib.F64, "f16s": _lib.F16S, "bf16s": _lib.BF16S}
+_PRECISIONS = {"f32": _lib.F32, "f64": _lib.F64, "f16s": _lib.F16S, "bf16s": _lib.BF16S,
+               "f16c": _lib.F16C}
 
 
 def precision_code(precision) -> int:
@@ -280,6 +281,11 @@ class LbmSolverHIP:
 
     def sync(self, stream=None):
         check(lib().lbm_sync(self._h, _stream(stream)))
+
+    def reset(self):
+        """Populations back to the rest equilibrium (flags kept); clears the
+        residual snapshot and the F16C overflow mark."""
+        check(lib().lbm_reset(self._h))
 
     # -- checkpoint of the whole A-A state ---------------------------------
     def save_state(self) -> np.ndarray:

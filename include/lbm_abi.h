@@ -59,12 +59,23 @@ enum lbm_direction {
 };
 
 /* Population storage. F32 and F64 reproduce the reference bit for bit;
- * F16S / BF16S store f_l - w_l in 16 bits with FP32 collision arithmetic. */
+ * the 16-bit formats store x = f_l - w_l (w_l the lattice weight) with FP32
+ * collision arithmetic:
+ *   F16S : IEEE binary16 of x * 2^12;
+ *   BF16S: bfloat16 of x;
+ *   F16C : 1 sign, 4 exponent, 11 significand bits of x, normal range
+ *          [2^-17, 2^-2) and subnormal step 2^-28 (FP16C of Lehmann et al.,
+ *          Phys. Rev. E 106, 015308, 2022). It has no infinities or NaN: a
+ *          store that does not fit (|x| >= 1/4 after rounding, or a
+ *          non-finite value) marks the solver, after which lbm_residual,
+ *          lbm_reduce_box and lbm_get_moments return NaN until lbm_reset or
+ *          lbm_load_state. */
 enum lbm_precision {
     LBM_F32 = 0,
     LBM_F64 = 1,
     LBM_F16S = 2,
-    LBM_BF16S = 3
+    LBM_BF16S = 3,
+    LBM_F16C = 4
 };
 
 enum lbm_status {

@@ -38,7 +38,7 @@ TUNINGS = [dict(block=256, cells_per_thread=1, waves_per_cu=0, options=(1, 2)),
            dict(block=256, cells_per_thread=1, waves_per_cu=0, options=(3, 0), pairs=4)]
 
 
-@pytest.mark.parametrize("precision", ["f32", "f64", "f16s", "bf16s"])
+@pytest.mark.parametrize("precision", ["f32", "f64", "f16s", "bf16s", "f16c"])
 @pytest.mark.parametrize("tuning", range(len(TUNINGS)))
 def test_batch_members_equal_solo_runs(hip, precision, tuning):
     from cavityflow_cuda_amd.batch import CavityBatch

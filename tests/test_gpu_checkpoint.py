@@ -22,7 +22,7 @@ def advance(s, first, count):
         (s.simulationStepBeta if i & 1 else s.simulationStepAlpha)()
 
 
-@pytest.mark.parametrize("precision", ["f32", "f64", "f16s", "bf16s"])
+@pytest.mark.parametrize("precision", ["f32", "f64", "f16s", "bf16s", "f16c"])
 @pytest.mark.parametrize("split", [40, 41])
 def test_resume_equals_uninterrupted_run(hip, precision, split):
     print("[case] checkpoint", precision, split, flush=True)

@@ -41,7 +41,7 @@ def run(s, full, alpha_opt, beta_opt):
     return a, b, s.getVelocities().copy(), s.getDensities().copy()
 
 
-@pytest.mark.parametrize("precision", ["f32", "f64", "f16s", "bf16s"])
+@pytest.mark.parametrize("precision", ["f32", "f64", "f16s", "bf16s", "f16c"])
 def test_every_launch_option_gives_the_same_bits(hip, precision):
     print("[case] launch options", precision, flush=True)
     want = None
@@ -57,7 +57,8 @@ def test_every_launch_option_gives_the_same_bits(hip, precision):
 
 
 # per-precision defaults (alpha, beta) documented in include/lbm_abi.h
-DEFAULTS = {"f32": (1 | 16, 16), "f64": (1 | 16, 128 | 16), "f16s": (1, 0), "bf16s": (1, 0)}
+DEFAULTS = {"f32": (1 | 16, 16), "f64": (1 | 16, 128 | 16), "f16s": (1, 0), "bf16s": (1, 0),
+            "f16c": (1, 0)}
 
 
 @pytest.mark.parametrize("precision", sorted(DEFAULTS))

@@ -241,7 +241,7 @@ def test_pack_unpack_faces_match_oracle(hip, oracle_cpu):
     s.close()
 
 
-@pytest.mark.parametrize("prec", ["f32", "f64", "f16s", "bf16s"])
+@pytest.mark.parametrize("prec", ["f32", "f64", "f16s", "bf16s", "f16c"])
 @pytest.mark.parametrize("size", [(9, 6, 7), (1, 4, 5)])
 def test_unpack_face_pair_equals_two_unpacks(hip, prec, size):
     """lbm_unpack_face_pair(axis) leaves exactly the state of the two

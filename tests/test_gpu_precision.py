@@ -1,12 +1,14 @@
-"""Shifted 16-bit population storage (f_l - w_l in FP16 / BF16, FP32 collision)
-against an FP64 run of the same kernels on the same cavity.
+"""Shifted 16-bit population storage (f_l - w_l in 16 bits, FP32 collision)
+against an FP64 run of the same kernels on the same cavity, in the relative L2
+norm of the interior velocity field, ||u - u64|| / ||u64||.
 
-Tolerances (relative L2 of the interior velocity field vs FP64). They are
-set to catch defects (a wrong shift, weight or slot gives errors of order 1),
-not to certify accuracy; the measured values are printed:
-  F32   <= 1e-3   (unshifted FP32, bit-exact with the reference; ~1e-4 measured)
-  F16S  <= 5e-3   (11-bit significand of f - w; ~1.3e-3 to 1.6e-3 measured)
-  BF16S <= 5e-2   (8-bit significand; reported)
+The bar for shifted 16-bit storage is 1e-3 on a cavity run to its steady
+state: 24^3, Re 100, lid 0.1, stepped until the FP64 run's residual
+||u^n - u^(n-100)|| / ||u^n|| falls to 1e-6. F16S and F16C must meet it; F16C
+(one more significand bit) must beat F16S. During a transient (32^3 at Re 400
+and 48x24x40 at Re 1000, far from steady) rounding differences are carried by
+the flow and 16-bit storage does not meet 1e-3; those runs are bounded at
+about 1.3x the errors measured on MI355X, to catch regressions, and printed.
 The FP64 run itself is bit-exact with the reference <double> kernels
 (tests/test_gpu_parity.py), which pins it as the oracle here.
 """
@@ -15,7 +17,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"f32": 1e-3, "f16s": 5e-3, "bf16s": 5e-2}
+BAR = 1e-3
 
 
 def run(hip, prec, size, u_lid, re, steps):
@@ -24,26 +26,63 @@ def run(hip, prec, size, u_lid, re, steps):
         c.computeNextStep()
     c.sync()
     u = c.getVelocities().astype(np.float64)
-    rho = c.getDensities().astype(np.float64)
     c.close()
-    return u, rho
+    return u
 
 
-@pytest.mark.parametrize("size,re,steps", [((32, 32, 32), 400.0, 2000), ((48, 24, 40), 1000.0, 1200)])
-def test_shifted_16bit_storage_error_vs_fp64(hip, size, re, steps):
-    print("[case] precision", size, re, steps, flush=True)
-    u64, r64 = run(hip, "f64", size, 0.1, re, steps)
-    norm = np.sqrt((u64 ** 2).sum())
-    assert norm > 0
+def rel_l2(u, u64):
+    return float(np.sqrt(((u - u64) ** 2).sum() / (u64 ** 2).sum()))
+
+
+def test_converged_cavity_error_vs_fp64(hip):
+    size, u_lid, re = (24, 24, 24), 0.1, 100.0
+    print("[case] converged precision", size, re, flush=True)
+    c = hip.CavityController(size, u_lid=u_lid, reynolds=re, precision="f64")
+    s = next(iter(c.solvers.values()))
+    steps, res = 0, float("inf")
+    while res > 1e-6 and steps < 20000:
+        for _ in range(100):
+            c.computeNextStep()
+        steps += 100
+        res = s.residual(layout=0)            # after beta: natural slots
+    c.sync()
+    u64 = c.getVelocities().astype(np.float64)
+    c.close()
+    print(f"  f64 residual {res:.2e} after {steps} steps", flush=True)
+    assert res <= 1e-6
     errs = {}
-    for prec in ("f32", "f16s", "bf16s"):
-        u, _ = run(hip, prec, size, 0.1, re, steps)
-        errs[prec] = np.sqrt(((u - u64) ** 2).sum()) / norm
-        linf = np.abs(u - u64).max()
-        print(f"  {prec}: rel L2 {errs[prec]:.3e}  Linf {linf:.3e}", flush=True)
+    for prec in ("f32", "f16s", "f16c", "bf16s"):
+        u = run(hip, prec, size, u_lid, re, steps)
         assert np.isfinite(u).all()
-        assert errs[prec] <= TOL[prec], (prec, errs[prec])
-    assert errs["f32"] < errs["f16s"]
+        errs[prec] = rel_l2(u, u64)
+        print(f"  {prec}: rel L2 {errs[prec]:.3e}  Linf {np.abs(u - u64).max():.3e}", flush=True)
+    assert errs["f32"] <= 1e-4
+    assert errs["f16s"] <= BAR
+    assert errs["f16c"] <= BAR
+    assert errs["f16c"] < errs["f16s"]
+    assert errs["bf16s"] <= 2e-2
+
+
+TRANSIENT = {((32, 32, 32), 400.0, 2000): {"f32": 2e-4, "f16s": 2e-3, "f16c": 1.8e-3,
+                                           "bf16s": 2e-2},
+             ((48, 24, 40), 1000.0, 1200): {"f32": 2e-4, "f16s": 2e-3, "f16c": 1.8e-3,
+                                            "bf16s": 2e-2}}
+
+
+@pytest.mark.parametrize("case", list(TRANSIENT))
+def test_transient_cavity_error_vs_fp64(hip, case):
+    size, re, steps = case
+    print("[case] transient precision", size, re, steps, flush=True)
+    u64 = run(hip, "f64", size, 0.1, re, steps)
+    assert np.sqrt((u64 ** 2).sum()) > 0
+    errs = {}
+    for prec, bound in TRANSIENT[case].items():
+        u = run(hip, prec, size, 0.1, re, steps)
+        assert np.isfinite(u).all()
+        errs[prec] = rel_l2(u, u64)
+        print(f"  {prec}: rel L2 {errs[prec]:.3e}  Linf {np.abs(u - u64).max():.3e}", flush=True)
+        assert errs[prec] <= bound, (prec, errs[prec])
+    assert errs["f32"] < errs["f16c"] < errs["f16s"] < errs["bf16s"]
 
 
 W = np.array([1 / 18] * 4 + [1 / 36] * 12 + [1 / 18] * 2 + [1 / 3], np.float32)
@@ -55,12 +94,13 @@ def _bf16_rne(x):
     return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
 
 
-@pytest.mark.parametrize("prec", ["f16s", "bf16s"])
+@pytest.mark.parametrize("prec", ["f16s", "bf16s", "f16c"])
 def test_shifted_codes_are_exact(hip, prec):
     """The device's 16-bit codes and their decode, bit for bit against numpy:
     F16S code = binary16 RNE of (f - w) * 2^12 (float32 arithmetic), decode =
     float32(code) * 2^-12 + w; BF16S code = bfloat16 RNE of f - w, decode =
-    float32(code << 16) + w."""
+    float32(code << 16) + w; F16C as restated in oracle/storage.py."""
+    from oracle import storage
     print("[case] shifted codes", prec, flush=True)
     s = hip.LbmSolverHIP((6, 5, 4), precision=prec)
     full = ((0, 0, 0), s.size_h)
@@ -76,6 +116,9 @@ def test_shifted_codes_are_exact(hip, prec):
         want = (d * np.float32(4096)).astype(np.float16).view(np.uint16)
         back = (raw.view(np.float16).astype(np.float32) * np.float32(2.0 ** -12)
                 + W[:, None]).astype(np.float32)
+    elif prec == "f16c":
+        want = storage.encode(f, "f16c")
+        back = storage.decode(raw, "f16c")
     else:
         want = _bf16_rne(d)
         back = ((raw.astype(np.uint32) << 16).view(np.float32) + W[:, None]).astype(np.float32)
@@ -89,7 +132,7 @@ def test_shifted_storage_round_trip(hip):
     """Rest equilibrium stores exactly zero deviations; set/get of f values
     round to the 16-bit grid of f - w."""
     print("[case] shifted round trip", flush=True)
-    for prec, eps in (("f16s", 2.0 ** -11), ("bf16s", 2.0 ** -8)):
+    for prec, eps in (("f16s", 2.0 ** -11), ("bf16s", 2.0 ** -8), ("f16c", 2.0 ** -12)):
         s = hip.LbmSolverHIP((6, 5, 4), precision=prec)
         full = ((0, 0, 0), s.size_h)
         d = s.getDensityDistributions(*full).reshape(19, -1)

@@ -48,7 +48,7 @@ def _noalloc(hip, solvers, boxes, buf):
     return buf.download(np.float64, 5 * n).reshape(n, 5)
 
 
-@pytest.mark.parametrize("prec", ["f32", "f64", "f16s", "bf16s"])
+@pytest.mark.parametrize("prec", ["f32", "f64", "f16s", "bf16s", "f16c"])
 def test_noalloc_reduce_equals_reduce_box(hip, prec):
     from hiprt import DeviceBuffer
     say("noalloc reduce", prec)
