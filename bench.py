@@ -273,7 +273,9 @@ def batched_workload(args, cf, L, hip, stream):
         for sp in specs:
             sp.params = cf.cavity_parameters(sp.size, sp.u_lid, sp.reynolds, prec)
         b = CavityBatch(specs, prec)
-        tuning = b.autotune(trial_steps=args.trial_steps, stream=stream.value)
+        # --no-graph: the trials too run as direct launches (a NULL stream is never captured)
+        tuning = b.autotune(trial_steps=args.trial_steps,
+                            stream=None if args.no_graph else stream.value)
         print(f"[batched] {prec} tuning {tuning}", file=sys.stderr, flush=True)
         b.run(100, stream=stream.value, graph=not args.no_graph)   # warm-up (graph capture)
         hip.sync(stream)
@@ -336,7 +338,9 @@ def serving_workload(args, cf, L, hip, stream):
     # capacity: the batch rate on a sample of the requests, with autotuned shapes
     sample = [CavitySpec(r.size, r.u_lid, r.reynolds, r.params) for r in reqs[:24]]
     b = CavityBatch(sample, "f32")
-    tuning = b.autotune(trial_steps=args.trial_steps, stream=stream.value)
+    # --no-graph: the trials too run as direct launches (a NULL stream is never captured)
+    tuning = b.autotune(trial_steps=args.trial_steps,
+                        stream=None if args.no_graph else stream.value)
     b.run(100, stream=stream.value, graph=not args.no_graph)
     hip.sync(stream)
     e0, e1 = hip.event(), hip.event()

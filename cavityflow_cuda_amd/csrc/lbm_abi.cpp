@@ -1023,6 +1023,18 @@ int lbm_load_state(lbm_solver* s, const void* src, uint64_t bytes)
     return state_copy(s, const_cast<void*>(src), bytes, false);
 }
 
+int lbm_device_count(int* count)
+{
+    if (!count) {
+        set_error("lbm_device_count: NULL argument");
+        return LBM_ERR_ARG;
+    }
+    int n = 0;
+    LBM_HIP(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    *count = n;
+    return LBM_OK;
+}
+
 int lbm_sync(lbm_solver* s, hipStream_t stream)
 {
     if (!s) {

@@ -169,6 +169,7 @@ struct lbm_batch {
     uint32_t pair_tiles = 0;
     int* check_flag = nullptr;         // boundary check result
     std::vector<GraphEntry> graphs;
+    std::vector<lbm_solver*> alt_owned;  // members whose dd_alt this batch allocated
 };
 
 namespace {
@@ -180,11 +181,30 @@ namespace {
             return hip_fail(e_, what);         \
     } while (0)
 
+// Destroys the cached graph executables once every launch of them has
+// finished (the tables their kernel nodes point to are about to be freed or
+// replaced).
 void drop_graphs(lbm_batch* b)
 {
-    for (auto& g : b->graphs)
+    for (auto& g : b->graphs) {
+        (void)hipStreamSynchronize(g.stream);
         (void)hipGraphExecDestroy(g.exec);
+    }
     b->graphs.clear();
+}
+
+// Frees the second population buffers this batch allocated for the pair pass
+// (after the device has finished with them).
+void free_alt(lbm_batch* b)
+{
+    if (b->alt_owned.empty())
+        return;
+    (void)hipDeviceSynchronize();
+    for (lbm_solver* s : b->alt_owned) {
+        (void)hipFree(s->dd_alt);
+        s->dd_alt = nullptr;
+    }
+    b->alt_owned.clear();
 }
 
 int upload_tiles(lbm_batch* b, uint32_t tile_cells, std::vector<MemberDesc>& desc,
@@ -286,10 +306,13 @@ int prepare_pairs(lbm_batch* b, int pairs)
             set_error("batch pair pass: a member has a non-OBSTACLE cell on its array boundary");
             return LBM_ERR_UNSUPPORTED;
         }
-        if (!s->dd_alt &&
-            hipMalloc(&s->dd_alt, (size_t)kQ * s->stride * s->elem_bytes) != hipSuccess) {
-            set_error("batch pair pass: cannot allocate the second population buffer");
-            return LBM_ERR_NOMEM;
+        if (!s->dd_alt) {
+            if (hipMalloc(&s->dd_alt, (size_t)kQ * s->stride * s->elem_bytes) != hipSuccess) {
+                s->dd_alt = nullptr;
+                set_error("batch pair pass: cannot allocate the second population buffer");
+                return LBM_ERR_NOMEM;
+            }
+            b->alt_owned.push_back(s);
         }
         PairDesc& d = desc[m];
         const MemberDesc& h = b->host_desc[m];
@@ -336,8 +359,13 @@ int apply_tuning(lbm_batch* b, const lbm_batch_tuning& t)
     drop_graphs(b);
     if (t.pairs) {
         int rc = prepare_pairs(b, t.pairs);
-        if (rc)
+        if (rc) {
+            if (!b->tuning.pairs)
+                free_alt(b);   // a failed check leaves no buffer behind
             return rc;
+        }
+    } else {
+        free_alt(b);           // no pair pass: no second population buffers
     }
     b->tuning = t;
     const uint32_t tile_cells = (uint32_t)(t.block * t.cells_per_thread);
@@ -580,6 +608,7 @@ int lbm_batch_destroy(lbm_batch* b)
     (void)set_dev(b->device);
     (void)hipDeviceSynchronize();
     drop_graphs(b);
+    free_alt(b);
     void* bufs[] = {b->desc,    b->tile_member, b->res_mem,     b->res_part,
                     b->res_out, b->pair_desc,   b->pair_member, b->check_flag};
     for (void* p : bufs)
@@ -669,14 +698,16 @@ int lbm_batch_run(lbm_batch* b, int first_phase, int half_steps, hipStream_t str
             BATCH_HIP(hipGraphLaunch(g.exec, stream), "hipGraphLaunch");
             return LBM_OK;
         }
-    hipGraph_t graph;
+    hipGraph_t graph = nullptr;
     BATCH_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
     const hipError_t le = enqueue(b, first_phase, half_steps, stream);
     hipError_t ee = hipStreamEndCapture(stream, &graph);
-    if (le != hipSuccess)
-        return hip_fail(le, "batch kernel capture");
-    if (ee != hipSuccess)
-        return hip_fail(ee, "hipStreamEndCapture");
+    if (le != hipSuccess || ee != hipSuccess) {
+        if (ee == hipSuccess && graph)
+            (void)hipGraphDestroy(graph);
+        return le != hipSuccess ? hip_fail(le, "batch kernel capture")
+                                : hip_fail(ee, "hipStreamEndCapture");
+    }
     GraphEntry g{first_phase, half_steps, stream, nullptr};
     hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);

@@ -189,15 +189,17 @@ inline double residual_ratio(double num, double den)
     return num > 0 ? std::numeric_limits<double>::infinity() : 0.0;
 }
 
-// The residual call's value: +inf on the first call after creation, reset or
-// load (no previous field to compare with) unless the state is already
-// non-finite, then residual_ratio.
+// The residual call's value: residual_ratio, except on the first call after
+// creation, reset or load, where no previous field exists (num compares with
+// a stale snapshot): +inf, or NaN when the current field (den) is already
+// non-finite.
 inline double first_residual(int valid, double num, double den)
 {
     if (valid)
         return residual_ratio(num, den);
-    const double r = residual_ratio(num, den);
-    return std::isnan(r) ? r : std::numeric_limits<double>::infinity();
+    if (std::isnan(den) || std::isinf(den))
+        return std::numeric_limits<double>::quiet_NaN();
+    return std::numeric_limits<double>::infinity();
 }
 
 }  // namespace lbm

@@ -338,8 +338,10 @@ class CavityServer:
             self.cell_updates += req.cells * self.K
             r = res[i]
             status = None
-            if not math.isfinite(r) and a[3] > self.K:
-                status = "failed"                 # the first check has no snapshot yet
+            # NaN: a non-finite state (any check); +inf: no snapshot yet (the
+            # first check), later a vanishing field
+            if math.isnan(r) or (math.isinf(r) and a[3] > self.K):
+                status = "failed"
             elif r <= req.tol:
                 status = "converged"
             elif a[3] >= req.max_steps:
@@ -440,7 +442,7 @@ def solve_sequential(requests, precision="f32", check_every=100, device=0, strea
                 check(L.lbm_simulation_step_beta(s.handle, None, None, st))
             steps += check_every
             r = s.residual(0, stream)
-            if not math.isfinite(r) and steps > check_every:
+            if math.isnan(r) or (math.isinf(r) and steps > check_every):
                 status = "failed"
                 break
             if r <= req.tol:

@@ -76,7 +76,8 @@ inline SolverParameters computeParameters(const lbm_physics& phys, int precision
 }
 
 // T is the host element type of population / velocity / density buffers:
-// double for LBM_F64 storage, float for LBM_F32, LBM_F16S and LBM_BF16S.
+// double for LBM_F64 storage, float for LBM_F32, LBM_F16S, LBM_BF16S and
+// LBM_F16C.
 template <typename T>
 class CLbmSolverHIP {
     static_assert(std::is_same<T, float>::value || std::is_same<T, double>::value,
@@ -136,26 +137,45 @@ public:
     }
 
     // get/setDensityDistributions (CLbmSolver.hpp:73-76, CLbmSolverCPU.hpp:80)
-    void getDensityDistributions(Vec3i& origin, Vec3i& size, T* dst)
+    // and their stream overloads (CLbmSolverGPU.cuh:73-82; nullptr = the
+    // default stream). With host buffers every call returns with its copy
+    // complete, ordered after the work already queued on the stream.
+    void getDensityDistributions(Vec3i& origin, Vec3i& size, T* dst, hipStream_t stream = nullptr)
     {
-        check(lbm_get_density_distributions(s_, origin.data, size.data, dst, 0, nullptr));
+        check(lbm_get_density_distributions(s_, origin.data, size.data, dst, 0, stream));
     }
-    void getDensityDistributions(T* dst)
+    void getDensityDistributions(T* dst, hipStream_t stream = nullptr)
     {
-        check(lbm_get_density_distributions(s_, nullptr, nullptr, dst, 0, nullptr));
+        check(lbm_get_density_distributions(s_, nullptr, nullptr, dst, 0, stream));
     }
-    void setDensityDistributions(Vec3i& origin, Vec3i& size, T* src)
+    void setDensityDistributions(Vec3i& origin, Vec3i& size, T* src, hipStream_t stream = nullptr)
     {
-        check(lbm_set_density_distributions(s_, origin.data, size.data, src, 0, nullptr));
+        check(lbm_set_density_distributions(s_, origin.data, size.data, src, 0, stream));
     }
-    void setDensityDistributions(Vec3i& origin, Vec3i& size, Direction direction, T* src)
+    void setDensityDistributions(Vec3i& origin, Vec3i& size, Direction direction, T* src,
+                                 hipStream_t stream = nullptr)
     {
         check(lbm_set_density_distributions_dir(s_, origin.data, size.data, direction, src, 0,
-                                                nullptr));
+                                                stream));
     }
-    void setDensityDistributions(T* src)
+    void setDensityDistributions(T* src, hipStream_t stream = nullptr)
     {
-        check(lbm_set_density_distributions(s_, nullptr, nullptr, src, 0, nullptr));
+        check(lbm_set_density_distributions(s_, nullptr, nullptr, src, 0, stream));
+    }
+
+    // Device-buffer forms of the population copies (asynchronous on the
+    // stream, as the reference GPU solver's device-side copies are): the
+    // halo exchange of a multi-GPU run passes its send / receive buffers here.
+    void getDensityDistributionsDevice(Vec3i& origin, Vec3i& size, T* dev_dst,
+                                       hipStream_t stream)
+    {
+        check(lbm_get_density_distributions(s_, origin.data, size.data, dev_dst, 1, stream));
+    }
+    void setDensityDistributionsDevice(Vec3i& origin, Vec3i& size, Direction direction,
+                                       const T* dev_src, hipStream_t stream)
+    {
+        check(lbm_set_density_distributions_dir(s_, origin.data, size.data, direction, dev_src,
+                                                1, stream));
     }
 
     // get/setFlags (CLbmSolver.hpp:77-80)

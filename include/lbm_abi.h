@@ -370,7 +370,11 @@ typedef struct lbm_batch_tuning {
                               buffer per member); needs F32 / F16S / BF16S, an
                               all-OBSTACLE array boundary and no stored rho / u
                               outputs. options[0] applies to the pass
-                              (NONTEMPORAL: streaming stores of the new state). */
+                              (NONTEMPORAL: streaming stores of the new state).
+                              The second buffers are the batch's: freed when a
+                              tuning without the pass is applied or the batch
+                              is destroyed, so a solver may be in the pair pass
+                              of one batch at a time. */
     int group_mb;          /* 0: every launch covers all members. > 0: members are
                               packed, in order, into groups of at most group_mb MiB
                               of populations (a larger member forms its own group);
@@ -401,8 +405,9 @@ int lbm_batch_launch(lbm_batch* batch, int first_phase, int half_steps, hipStrea
 /* Times candidate tunings (block, cells_per_thread, waves_per_cu, launch
  * options, member groups with the shapes again under the chosen group, and
  * the pair pass when the members allow it) as runs of trial_steps whole steps
- * (graph-captured on a stream) and keeps the fastest; the winner must also
- * beat the starting tuning over longer runs.
+ * (lbm_batch_run: graph-captured on a non-NULL stream, direct launches on the
+ * NULL stream) and keeps the fastest; the winner must also beat the starting
+ * tuning over longer runs.
  * The trials are real steps: the members advance; reset them afterwards
  * (lbm_reset) when their state matters. Synchronous. */
 int lbm_batch_autotune(lbm_batch* batch, int trial_steps, hipStream_t stream,
@@ -436,6 +441,11 @@ int lbm_load_state(lbm_solver* solver, const void* src, uint64_t bytes);
 int lbm_device_bytes(const lbm_params* params, int pair_buffer, uint64_t* bytes);
 
 /* ---- misc ---------------------------------------------------------------- */
+
+/* Number of visible GPUs: cudaGetDeviceCount in the CLbmSolverGPU constructor
+ * (CLbmSolverGPU.cu:43-45), which places subdomain `id` on device
+ * id % count. */
+int lbm_device_count(int* count);
 
 int lbm_sync(lbm_solver* solver, hipStream_t stream);
 const char* lbm_last_error(void);

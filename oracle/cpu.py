@@ -57,6 +57,8 @@ def _declare(L):
         getattr(L, f"oracle_solver_params_{s}").argtypes = [ct, ctypes.c_int, ct, _P, _P, ct, ct,
                                                             ct, ct, _P, _P, _P, _P, _P, _P]
         getattr(L, f"oracle_solver_params_{s}").restype = ctypes.c_int
+        getattr(L, f"oracle_moments_{s}").argtypes = [_P, _P, _P, _P, ctypes.c_int, _P]
+        getattr(L, f"oracle_moments_{s}").restype = None
         for fn in ("init", "alpha", "beta", "get_box", "set_box"):
             getattr(L, f"oracle_{fn}_{s}").restype = None
 
@@ -192,6 +194,15 @@ class OracleSolver:
     def getDensities(self, origin=None, size=None):
         o, b = ((1, 1, 1), self.size) if origin is None else (origin, size)
         return self.get_box(self.rho, 1, o, b)
+
+    def moments(self, layout: int, origin=None, size=None) -> np.ndarray:
+        """[rho, jx, jy, jz] of every cell of the box (default: interior) from
+        the populations, in the reference accumulation order."""
+        o, b = ((1, 1, 1), self.size) if origin is None else (origin, size)
+        dst = np.empty(4 * int(np.prod(b)), dtype=self.dtype)
+        getattr(lib(), f"oracle_moments_{self.s}")(_ptr(self.dd), _i3(*self.size_h), _i3(*o),
+                                                  _i3(*b), int(layout), _ptr(dst))
+        return dst
 
     def set_driven_cavity(self):
         """CController::setDrivenCavitySzenario (CController.cpp:1313-1345)."""

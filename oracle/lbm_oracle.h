@@ -60,6 +60,8 @@ void oracle_set_flags(int32_t* flags, const int size_h[3], const int bc[6], int 
     void oracle_set_box_##S(T* field, int ncomp, const int size_h[3],                 \
                             const int origin[3], const int box[3], const T* src,      \
                             uint32_t comp_mask);                                      \
+    void oracle_moments_##S(const T* dd, const int size_h[3], const int origin[3],    \
+                            const int box[3], int layout, T* dst);                    \
     int oracle_solver_params_##S(T length_x, int size_x, T length_max,                \
                                  const T velocity[3], const T acceleration[3],        \
                                  T viscosity, T timestep, T max_velocity,             \

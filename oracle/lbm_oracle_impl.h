@@ -604,6 +604,33 @@ void FN(oracle_set_box)(T* field, int ncomp, const int size_h[3], const int orig
     }
 }
 
+/* Density and momentum of every cell of a box from its populations, with the
+ * accumulation of CLbmAlphaCPU.cpp:87-182 (the `moments` above, which the
+ * alpha and beta cells use): dst[4][z][y][x] = rho, jx, jy, jz. layout 0: slot
+ * l holds direction l (after beta or init); 1: slot l holds direction opp(l)
+ * (after alpha). */
+void FN(oracle_moments)(const T* dd, const int size_h[3], const int origin[3], const int box[3],
+                        int layout, T* dst)
+{
+    long nx = size_h[0], nxy = (long)size_h[0] * size_h[1], n = nxy * size_h[2];
+    long cells = (long)box[0] * box[1] * box[2];
+    int z, y, x, l;
+    for (z = 0; z < box[2]; z++)
+        for (y = 0; y < box[1]; y++)
+            for (x = 0; x < box[0]; x++) {
+                long id = (long)(origin[2] + z) * nxy + (long)(origin[1] + y) * nx + (origin[0] + x);
+                long g = ((long)z * box[1] + y) * box[0] + x;
+                T d[19], r, jx, jy, jz;
+                for (l = 0; l < 19; l++)
+                    d[layout ? (l < 18 ? (l ^ 1) : 18) : l] = dd[l * n + id];
+                FN(moments)(d, &r, &jx, &jy, &jz);
+                dst[g] = r;
+                dst[cells + g] = jx;
+                dst[2 * cells + g] = jy;
+                dst[3 * cells + g] = jz;
+            }
+}
+
 static T FN(length3)(const T v[3])
 {
     return FN(oracle_sqrt)(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);

@@ -56,17 +56,20 @@ def test_converged_cavity_error_vs_fp64(hip):
         assert np.isfinite(u).all()
         errs[prec] = rel_l2(u, u64)
         print(f"  {prec}: rel L2 {errs[prec]:.3e}  Linf {np.abs(u - u64).max():.3e}", flush=True)
+    # measured on MI355X: 2.6e-5, 5.4e-4, 3.8e-4, 1.6e-3 after 4000 steps
     assert errs["f32"] <= 1e-4
     assert errs["f16s"] <= BAR
     assert errs["f16c"] <= BAR
     assert errs["f16c"] < errs["f16s"]
-    assert errs["bf16s"] <= 2e-2
+    assert errs["bf16s"] <= 2.5e-3
 
 
-TRANSIENT = {((32, 32, 32), 400.0, 2000): {"f32": 2e-4, "f16s": 2e-3, "f16c": 1.8e-3,
-                                           "bf16s": 2e-2},
-             ((48, 24, 40), 1000.0, 1200): {"f32": 2e-4, "f16s": 2e-3, "f16c": 1.8e-3,
-                                            "bf16s": 2e-2}}
+# measured on MI355X: 9.2e-5 / 1.56e-3 / 1.18e-3 / 3.8e-3 and
+# 8.0e-5 / 1.31e-3 / 9.6e-4 / 3.3e-3
+TRANSIENT = {((32, 32, 32), 400.0, 2000): {"f32": 1.2e-4, "f16s": 2e-3, "f16c": 1.5e-3,
+                                           "bf16s": 5e-3},
+             ((48, 24, 40), 1000.0, 1200): {"f32": 1.1e-4, "f16s": 1.7e-3, "f16c": 1.25e-3,
+                                            "bf16s": 4.3e-3}}
 
 
 @pytest.mark.parametrize("case", list(TRANSIENT))

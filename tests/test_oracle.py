@@ -245,3 +245,53 @@ def test_direction_masks(oracle_cpu):
     for d, ls in expected.items():
         m = oracle_cpu.direction_mask(d)
         assert {l for l in range(19) if (m >> l) & 1} == ls
+
+
+# accumulation sequences of CLbmAlphaCPU.cpp:87-182, written out from the
+# reference's statement order: (slot, sign) per step after the first term
+_JX = [(0, 1), (1, -1), (4, 1), (5, -1), (6, 1), (7, -1), (8, 1), (9, -1), (10, 1), (11, -1)]
+_JY = [(2, 1), (3, -1), (4, 1), (5, -1), (6, -1), (7, 1), (12, 1), (13, -1), (14, 1), (15, -1)]
+_JZ = [(8, 1), (9, -1), (10, -1), (11, 1), (12, 1), (13, -1), (14, -1), (15, 1), (16, 1),
+       (17, -1)]
+
+
+def _sequential(d, seq, dtype):
+    acc = d[seq[0][0]].astype(dtype).copy()
+    for l, sign in seq[1:]:
+        acc = (acc + d[l]).astype(dtype) if sign > 0 else (acc - d[l]).astype(dtype)
+    return acc
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_moments_follow_the_reference_order(oracle_cpu, dtype):
+    """oracle_moments (the checker of lbm_get_moments): rho is the index-order
+    sum, j the reference's statement order, equal bit for bit to the momentum
+    alpha stores for FLUID cells (alpha is pinned by the golden vectors); the
+    swapped layout of an opp-permuted state gives the same bits."""
+    size = (7, 5, 6)
+    rng = np.random.default_rng(21)
+    o = oracle_cpu.OracleSolver(size, (OBST,) * 6, dtype=dtype, inv_tau=1.3)
+    o.dd[:] = (rng.random(o.dd.size) * 0.08 + 0.01).astype(dtype)
+    o.flags[:] = rng.choice([OBST, FLUID, INJ, GHOST], size=o.n).astype(np.int32)
+    full = ((0, 0, 0), o.size_h)
+    m = o.moments(0, *full).reshape(4, -1)
+    d = o.dd.reshape(19, -1)
+    assert m[0].tobytes() == _sequential(d, [(l, 1) for l in range(19)], dtype).tobytes()
+    for k, seq in ((1, _JX), (2, _JY), (3, _JZ)):
+        assert m[k].tobytes() == _sequential(d, seq, dtype).tobytes()
+    swapped = o.dd.reshape(19, -1)[OPP].copy()
+    keep = o.dd.copy()
+    o.dd[:] = swapped.reshape(-1)
+    assert o.moments(1, *full).tobytes() == m.tobytes()
+    o.dd[:] = keep
+    o.alpha()
+    u = o.u.reshape(3, -1)
+    fluid = o.flags == FLUID
+    for k in range(3):
+        assert u[k][fluid].tobytes() == m[k + 1][fluid].tobytes()
+    # a box is the same cells' values
+    box = ((1, 2, 0), (4, 2, 5))
+    mb = oracle_cpu.OracleSolver(size, (OBST,) * 6, dtype=dtype)
+    mb.dd[:] = keep
+    want = m.reshape(4, o.size_h[2], o.size_h[1], o.size_h[0])[:, 0:5, 2:4, 1:5]
+    assert mb.moments(0, *box).tobytes() == np.ascontiguousarray(want).tobytes()
