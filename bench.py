@@ -174,10 +174,15 @@ def cpu_baseline(size, params, sample_steps, threads_all):
                              store_densities=False, store_velocities=False,
                              nthreads=threads_all)
         o.set_driven_cavity()
+        # a sample of about 10 s: as many whole steps as the single-thread
+        # rate suggests, at least the single-thread sample
+        steps_all = max(sample_steps, 2 * int(5.0 * threads_all / max(dt / sample_steps, 1e-3)))
+        steps_all = min(steps_all, 400)
         t0 = time.perf_counter()
-        for n in range(sample_steps):
+        for n in range(steps_all):
             o.beta() if n & 1 else o.alpha()
         dt = time.perf_counter() - t0
+        sample_steps = steps_all
         out["all"] = dict(value=round(cells * sample_steps / dt / 1e6, 3), unit="MLUPS",
                           cores=threads_all, kind="port", **host,
                           sample=f"{sample_steps} steps, oracle C restatement, OpenMP over z, "
@@ -195,7 +200,7 @@ def precision_workload(args, cf, L, hip, stream):
     cells = int(np.prod(size))
     runs = {}
     j64 = None
-    for prec in ("f64", "f32", "f16s", "bf16s"):
+    for prec in ("f64", "f32", "f16s", "f16c", "bf16s"):
         params = cf.cavity_parameters(size, args.u_lid, re, prec)
         c = cf.CavityController(size, u_lid=args.u_lid, reynolds=re, precision=prec,
                                 store_outputs=False, params=params, stream=stream.value)
@@ -252,6 +257,7 @@ def precision_workload(args, cf, L, hip, stream):
                    "reynolds": re},
         "runs": runs,
         "mlups_ratio_f16s_over_f32": round(runs["f16s"]["mlups"] / runs["f32"]["mlups"], 3),
+        "mlups_ratio_f16c_over_f32": round(runs["f16c"]["mlups"] / runs["f32"]["mlups"], 3),
     }
     print(json.dumps(line), flush=True)
 
@@ -269,7 +275,7 @@ def batched_workload(args, cf, L, hip, stream):
     cells = sum(sp.cells for sp in specs)
     print(f"[batched] {len(specs)} cavities, {cells / 1e6:.2f} M interior cells, "
           f"{redrawn} draws redrawn, {steps} steps", file=sys.stderr, flush=True)
-    for prec in ("f32", "f16s"):
+    for prec in ("f32", "f16s", "f16c"):
         for sp in specs:
             sp.params = cf.cavity_parameters(sp.size, sp.u_lid, sp.reynolds, prec)
         b = CavityBatch(specs, prec)
@@ -330,14 +336,16 @@ def serving_workload(args, cf, L, hip, stream):
     under an HBM budget."""
     from cavityflow_cuda_amd.batch import CavityBatch, CavitySpec
     from cavityflow_cuda_amd import serving as sv
-    reqs, redrawn = sv.draw_requests(args.requests, args.seed, max_steps=args.max_steps)
+    prec = args.precision
+    reqs, redrawn = sv.draw_requests(args.requests, args.seed, max_steps=args.max_steps,
+                                     precision=prec)
     work = np.array([r.cells * r.max_steps for r in reqs], dtype=np.float64)
     print(f"[serving] {len(reqs)} requests, mean {np.mean([r.cells for r in reqs]) / 1e6:.2f} M "
           f"cells, cap {args.max_steps} steps, {work.sum() / 1e12:.2f} T cell updates at the cap",
           file=sys.stderr, flush=True)
     # capacity: the batch rate on a sample of the requests, with autotuned shapes
     sample = [CavitySpec(r.size, r.u_lid, r.reynolds, r.params) for r in reqs[:24]]
-    b = CavityBatch(sample, "f32")
+    b = CavityBatch(sample, prec)
     # --no-graph: the trials too run as direct launches (a NULL stream is never captured)
     tuning = b.autotune(trial_steps=args.trial_steps,
                         stream=None if args.no_graph else stream.value)
@@ -363,7 +371,7 @@ def serving_workload(args, cf, L, hip, stream):
               f"queued {len(srv.queue)} resident {srv.used / 1e9:.1f} GB", file=sys.stderr,
               flush=True)
 
-    srv = sv.CavityServer("f32", hbm_budget=args.hbm_budget * 1e9, check_every=100,
+    srv = sv.CavityServer(prec, hbm_budget=args.hbm_budget * 1e9, check_every=100,
                           stream=stream.value, tuning=tuning, policy=args.policy,
                           graph=not args.no_graph)
     served = srv.run(reqs, progress=prog)
@@ -378,11 +386,25 @@ def serving_workload(args, cf, L, hip, stream):
     def prog2(done, total):
         print(f"[serving] sequential done {done}/{total}", file=sys.stderr, flush=True)
 
-    plain = sv.solve_sequential(reqs, "f32", 100, stream=stream.value, progress=prog2)
+    plain = sv.solve_sequential(reqs, prec, 100, stream=stream.value, progress=prog2)
     p_sum = sv.latency_summary(plain)
     p_lu = sum(reqs[i].cells * r.steps for i, r in plain.items() if r.status != "rejected")
     p_mlups = p_lu / p_sum["span_s"] / 1e6
-    print(f"[serving] plain {p_sum}", file=sys.stderr, flush=True)
+    print(f"[serving] plain {prec} {p_sum}", file=sys.stderr, flush=True)
+    # shifted storage is also compared with plain sequential solves in FP32, the
+    # reference's precision (the same trace, its own CLbmSolver parameters)
+    f32 = None
+    if prec != "f32":
+        reqs32, _ = sv.draw_requests(args.requests, args.seed, max_steps=args.max_steps)
+        for r, r0 in zip(reqs32, reqs):
+            r.arrival = r0.arrival
+        plain32 = sv.solve_sequential(reqs32, "f32", 100, stream=stream.value, progress=prog2)
+        f_sum = sv.latency_summary(plain32)
+        f_lu = sum(reqs32[i].cells * r.steps for i, r in plain32.items() if r.status != "rejected")
+        f32 = dict(f_sum, aggregate_mlups=round(f_lu / f_sum["span_s"] / 1e6, 1))
+        f32["throughput_ratio_served_over_this"] = round(s_sum["solves_per_s"] / f_sum["solves_per_s"], 3)
+        f32["p99_ratio_this_over_served"] = round(f_sum["p99_s"] / s_sum["p99_s"], 3)
+        print(f"[serving] plain f32 {f_sum}", file=sys.stderr, flush=True)
     same = sum(int(served[i].steps == plain[i].steps and
                    served[i].totals.tobytes() == plain[i].totals.tobytes()) for i in served)
     statuses = {}
@@ -391,9 +413,9 @@ def serving_workload(args, cf, L, hip, stream):
     line = {
         "metric": "served cavity solves/s (p50/p99 latency) vs plain sequential",
         "value": round(s_sum["solves_per_s"], 4), "unit": "solves/s", "n_gpus": 1,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": prec,
         "data": "synthetic",
-        "config": {"workload": f"{len(reqs)} cavity solve requests, " +
+        "config": {"workload": f"{len(reqs)} cavity solve requests, {prec} storage, " +
                                (f"Poisson arrivals at {args.load:.0%} of measured capacity"
                                 if args.load > 0 else "all arriving at once") +
                                ", edges log-uniform [48, 384] "
@@ -405,6 +427,7 @@ def serving_workload(args, cf, L, hip, stream):
         "served": dict(s_sum, aggregate_mlups=round(s_mlups, 1), peak_resident_gb=round(peak_gb, 2),
                        rounds=rounds, statuses=statuses),
         "plain_sequential": dict(p_sum, aggregate_mlups=round(p_mlups, 1)),
+        "plain_sequential_f32": f32,
         "throughput_ratio": round(s_sum["solves_per_s"] / p_sum["solves_per_s"], 3),
         "mlups_ratio": round(s_mlups / p_mlups, 3),
         "p99_ratio_plain_over_served": round(p_sum["p99_s"] / s_sum["p99_s"], 3),
@@ -567,6 +590,102 @@ def step_run(args, cf, L, hip, stream, precision, steps, warmup):
                 wall=wall, halo_ms=halo_ms, step_ms_total=step_ms_total, steps=steps)
 
 
+def ceiling_workload(args, cf, L, hip, stream):
+    """Bandwidth ceilings of the step kernels' access patterns at the bench
+    size (FP32): the library's pattern probes (lbm_probe_pattern: alpha's
+    aligned 19-plane read-modify-write, beta's shifted one, 19-plane read and
+    write streams) under the launch options the steps use, a device-to-device
+    copy, and the step kernels themselves (no exchange) for comparison. Each
+    figure is bytes moved / mean launch time over `reps` launches timed with
+    HIP events on the launch stream."""
+    from cavityflow_cuda_amd._lib import check
+    size = (args.size,) * 3
+    reps = 10
+    s = cf.LbmSolverHIP(size, (cf.OBSTACLE,) * 6, "f32", inv_tau=1.0 / 0.6536,
+                        lid_velocity=args.u_lid, store_densities=False, store_velocities=False)
+    s.setDrivenCavitySzenario()
+    e0, e1 = hip.event(), hip.event()
+
+    def timed(launch):
+        launch()
+        hip.sync(stream)
+        hip.record(e0, stream)
+        for _ in range(reps):
+            launch()
+        hip.record(e1, stream)
+        return hip.elapsed_ms(e0, e1) / reps
+
+    names = {0: "rmw_aligned", 1: "rmw_shifted", 2: "read", 3: "write",
+             4: "rmw_shifted_pitched", 5: "rmw_shifted_x", 6: "rmw_shifted_yz"}
+    vec = [256, 512, 256 | 16, 512 | 16, 256 | 8, 512 | 8, 256 | 4, 512 | 4, 512 | 1 | 16]
+    opt_sets = {0: [0, 1, 16, 17, 8, 9, 4] + vec, 1: [0, 1, 16, 17, 8, 9, 4] + vec,
+                2: [0, 1, 16], 3: [0, 1, 16], 4: [0, 16, 8, 4], 5: [0, 16, 8, 4],
+                6: [0, 16, 8, 4]}
+    probes = []
+    for mode in (2, 3, 0, 1, 4, 5, 6):
+        for opt in opt_sets[mode]:
+            cells = ctypes.c_uint64()
+            check(L.lbm_probe_pattern(s.handle, mode, opt, ctypes.byref(cells), stream))
+            ms = timed(lambda: check(L.lbm_probe_pattern(s.handle, mode, opt, None, stream)))
+            nbytes = cells.value * 19 * 4 * (1 if mode in (2, 3) else 2)
+            probes.append(dict(pattern=names[mode], options=opt, ms=round(ms, 4),
+                               gbs=round(nbytes / (ms * 1e-3) / 1e9, 1)))
+            print(f"[ceiling] {probes[-1]}", file=sys.stderr, flush=True)
+    # the step kernels (whole domain, library default options, no exchange)
+    s.reset()
+    steps = {}
+    for k, fn in ((0, L.lbm_simulation_step_alpha), (1, L.lbm_simulation_step_beta)):
+        v = ctypes.c_int()
+        check(L.lbm_get_launch_options(s.handle, k, ctypes.byref(v)))
+
+        def pair():
+            check(L.lbm_simulation_step_alpha(s.handle, None, None, stream))
+            check(L.lbm_simulation_step_beta(s.handle, None, None, stream))
+        pair()
+        hip.sync(stream)
+        ev = [(hip.event(), hip.event()) for _ in range(reps)]
+        for a, b in ev:
+            if k == 0:
+                hip.record(a, stream)
+                check(L.lbm_simulation_step_alpha(s.handle, None, None, stream))
+                hip.record(b, stream)
+                check(L.lbm_simulation_step_beta(s.handle, None, None, stream))
+            else:
+                check(L.lbm_simulation_step_alpha(s.handle, None, None, stream))
+                hip.record(a, stream)
+                check(L.lbm_simulation_step_beta(s.handle, None, None, stream))
+                hip.record(b, stream)
+        ms = float(np.mean([hip.elapsed_ms(a, b) for a, b in ev]))
+        nbytes = int(np.prod(size)) * 2 * 19 * 4
+        steps["k_alpha" if k == 0 else "k_beta"] = dict(options=v.value, ms=round(ms, 4),
+                                                      gbs=round(nbytes / (ms * 1e-3) / 1e9, 1))
+    print(f"[ceiling] steps {steps}", file=sys.stderr, flush=True)
+    s.close()
+    copy = round(copy_bandwidth(hip, L, 0, stream), 1)
+    best = {}
+    for p in probes:
+        if p["gbs"] > best.get(p["pattern"], {"gbs": 0})["gbs"]:
+            best[p["pattern"]] = p
+    line = {
+        "metric": "HBM ceilings of the step kernels' access patterns",
+        "value": best["rmw_shifted"]["gbs"], "unit": "GB/s", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{size[0]}^3 interior ({size[0] + 2}^3 with walls) FP32 "
+                               "population array, 19 planes, one thread per cell",
+                   "reps": reps},
+        "peak_gbs": HBM_PEAK_GBS,
+        "device_copy_gbs": copy,
+        "best": {k: dict(v, frac=round(v["gbs"] / HBM_PEAK_GBS, 4)) for k, v in best.items()},
+        "probes": probes,
+        "step_kernels": {k: dict(v, frac=round(v["gbs"] / HBM_PEAK_GBS, 4),
+                                 of_pattern_ceiling=round(
+                                     v["gbs"] / best["rmw_aligned" if k == "k_alpha"
+                                                     else "rmw_shifted"]["gbs"], 4))
+                         for k, v in steps.items()},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -599,7 +718,7 @@ def main():
                     help="LBM_LAUNCH_* bits for the alpha kernel (default: library default)")
     ap.add_argument("--launch-beta", type=int, default=None)
     ap.add_argument("--workload", default="step",
-                    choices=("step", "precision", "batched", "serving"))
+                    choices=("step", "precision", "batched", "serving", "ceiling"))
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--cavities", type=int, default=64, help="batched: number of cavities")
     ap.add_argument("--no-graph", action="store_true",
@@ -642,6 +761,9 @@ def main():
         return
     if args.workload == "serving":
         serving_workload(args, cf, L, hip, stream)
+        return
+    if args.workload == "ceiling":
+        ceiling_workload(args, cf, L, hip, stream)
         return
     run = step_run(args, cf, L, hip, stream, args.precision, args.steps, args.warmup)
     size, params, mlups, total_ms = run["size"], run["params"], run["mlups"], run["total_ms"]

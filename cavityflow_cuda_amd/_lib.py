@@ -40,7 +40,7 @@ EXPORTED = (
     "lbm_batch_residual", "lbm_reset", "lbm_device_bytes", "lbm_state_bytes",
     "lbm_save_state", "lbm_load_state",
     "lbm_reduce_box", "lbm_reduce_boxes_noalloc", "lbm_residual", "lbm_sync", "lbm_last_error", "lbm_status_string",
-    "lbm_abi_version", "lbm_device_count",
+    "lbm_abi_version", "lbm_device_count", "lbm_probe_pattern",
 )
 
 
@@ -186,6 +186,7 @@ def lib() -> ctypes.CDLL:
         "lbm_status_string": ([I], ctypes.c_char_p),
         "lbm_abi_version": ([], I),
         "lbm_device_count": ([ctypes.POINTER(I)], I),
+        "lbm_probe_pattern": ([P, I, I, ctypes.POINTER(ctypes.c_uint64), P], I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -18,7 +18,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB_NAME = "libcavityflow_hip.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
 
-SOURCES = ["lbm_step.hip", "lbm_box.hip", "lbm_reduce.hip", "lbm_batch.hip", "lbm_abi.cpp"]
+SOURCES = ["lbm_step.hip", "lbm_box.hip", "lbm_reduce.hip", "lbm_batch.hip", "lbm_probe.hip",
+           "lbm_abi.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

@@ -101,11 +101,12 @@ struct StoreBF16S : Unchecked<StoreBF16S> {
 // at lid speed 0.1); no infinities or NaN, so a store that rounds to
 // |x| >= 2^-2, or of a non-finite value, is reported through store_checked.
 //
-// Encode: y = x * 2^-109 moves the exponent so that our e is the float's
+// Encode: |y| = |x| * 2^-109 moves the exponent so that our e is the float's
 // exponent field (2^(e - 18) * 2^-109 = 2^(e - 127)) and our subnormals are
 // float subnormals; the multiplication is exact for every x = f - w with w >=
 // 1/36 (x is then a multiple of 2^-29), and the 12 low bits of |y| are
-// rounded to nearest even in integer arithmetic (the carry moves into e).
+// rounded to nearest even in integer arithmetic (the carry moves into e); the
+// code is the sign of x over the rounded magnitude.
 // Decode: the code shifted left by 12 is y again (sign-extending the 16-bit
 // code copies the sign into bits 27..31, the mask keeps bit 31), and one FMA
 // by 2^109 gives w + x with one rounding. FP32 subnormals are kept on gfx950
@@ -123,11 +124,14 @@ struct StoreF16C {
     }
     __device__ static __forceinline__ S store_checked(C v, int l, uint32_t& ov)
     {
-        const uint32_t y = __float_as_uint((v - weight<float>(l)) * 0x1p-109f);
-        const uint32_t a = y & 0x7FFFFFFFu;
+        const float x = v - weight<float>(l);
+        const uint32_t a = __float_as_uint(__builtin_fabsf(x) * 0x1p-109f);
         const uint32_t r = a + 0x7FFu + ((a >> 12) & 1u);
         ov |= r;
-        return (S)(((y >> 16) & 0x8000u) | ((r >> 12) & 0x7FFFu));
+        // sign in bit 31, the rounded magnitude in bits 16..30: the code is
+        // the high half
+        const uint32_t hi = (__float_as_uint(x) & 0x80000000u) | (r << 4);
+        return (S)(hi >> 16);
     }
     __device__ static __forceinline__ S store(C v, int l)
     {

@@ -440,6 +440,32 @@ int lbm_load_state(lbm_solver* solver, const void* src, uint64_t bytes);
  * population buffer of the batch pair pass. */
 int lbm_device_bytes(const lbm_params* params, int pair_buffer, uint64_t* bytes);
 
+/* ---- measurement --------------------------------------------------------
+ * No reference counterpart: bandwidth probes of the step kernels' access
+ * patterns without the collision, the ceilings that bound k_alpha / k_beta on
+ * the device (bench.py --workload ceiling). They run on an FP32 solver's own
+ * population array, one thread per cell: */
+enum lbm_probe_mode {
+    LBM_PROBE_RMW_ALIGNED = 0,  /* load + store the cell's own 19 slots (alpha's pattern) */
+    LBM_PROBE_RMW_SHIFTED = 1,  /* load + store plane l at id + c_l (beta's pattern), cells
+                                   whose neighbours lie inside the array */
+    LBM_PROBE_READ = 2,         /* 19 loads per cell */
+    LBM_PROBE_WRITE = 3,        /* 19 stores per cell (stores the rest state) */
+    LBM_PROBE_RMW_SHIFTED_PITCHED = 4, /* beta's pattern with rows padded to 128 bytes:
+                                   shifts c_x + c_y P + c_z P ny_h, P = nx_h rounded
+                                   up to 32 */
+    LBM_PROBE_RMW_SHIFTED_X = 5,  /* the x components of beta's shifts only */
+    LBM_PROBE_RMW_SHIFTED_YZ = 6  /* the y and z components only */
+};
+/* options: LBM_LAUNCH_NONTEMPORAL and LBM_LAUNCH_WAVES4/3/2, and for the
+ * read-modify-write patterns LBM_PROBE_VEC2 / LBM_PROBE_VEC4 (2 or 4
+ * consecutive cells per thread, one vector access per plane); *cells (may be
+ * NULL) receives the number of cells the launch covers. Asynchronous. */
+#define LBM_PROBE_VEC2 256
+#define LBM_PROBE_VEC4 512
+int lbm_probe_pattern(lbm_solver* solver, int mode, int options, uint64_t* cells,
+                      hipStream_t stream);
+
 /* ---- misc ---------------------------------------------------------------- */
 
 /* Number of visible GPUs: cudaGetDeviceCount in the CLbmSolverGPU constructor

@@ -13,10 +13,10 @@ diagonals, 1/3 at rest), and collides in FP32:
   BF16S: bfloat16 of x (round to nearest even); decode float32(code << 16) + w.
   F16C : 1 sign, 4 exponent, 11 significand bits of x (FP16C, Lehmann et al.,
          Phys. Rev. E 106, 015308, 2022) with e = 1..15 -> 2^(e-18)(1 + m/2^11)
-         and e = 0 -> m 2^-28; no infinities or NaN. Encode: y = x * 2^-109 in
-         float32 (subnormal results kept), then the 12 low bits of |y| rounded to
-         nearest even in integer arithmetic; decode (code << 12 with the sign
-         kept) * 2^109 + w with one rounding.
+         and e = 0 -> m 2^-28; no infinities or NaN. Encode: |y| = |x| * 2^-109
+         in float32 (subnormal results kept), its 12 low bits rounded to nearest
+         even in integer arithmetic, the sign of x on top; decode (code << 12
+         with the sign kept) * 2^109 + w with one rounding.
 
 Pinning (tests/test_storage_formats.py): F16S and BF16S against numpy's own
 float16 and an independent float64 rounding; F16C exhaustively over all 65536
@@ -44,10 +44,6 @@ C = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [1, 1, 0], [-1, -1, 
 
 PRECISIONS = ("f16s", "bf16s", "f16c")
 F16C_OVERFLOW = 0x08000000
-
-
-def _w(planes):
-    return W[:, None] if planes == 19 else W[:planes, None]
 
 
 def decode(codes: np.ndarray, prec: str) -> np.ndarray:
@@ -91,11 +87,12 @@ def f16c_encode_x(x: np.ndarray):
     """float32 x -> (F16C codes, rounded |y| words); word >= F16C_OVERFLOW
     means out of range."""
     x = np.asarray(x, np.float32)
-    y = (x.astype(np.float64) * 2.0 ** -109).astype(np.float32).view(np.uint32)
-    a = y & np.uint32(0x7FFFFFFF)
+    a = (np.abs(x).astype(np.float64) * 2.0 ** -109).astype(np.float32).view(np.uint32)
     r = a + np.uint32(0x7FF) + ((a >> np.uint32(12)) & np.uint32(1))
-    code = ((y >> np.uint32(16)) & np.uint32(0x8000)) | ((r >> np.uint32(12)) & np.uint32(0x7FFF))
-    return code.astype(np.uint16), r
+    # the sign in bit 31 and the rounded magnitude in bits 16..30: the code is
+    # the high half
+    hi = (x.view(np.uint32) & np.uint32(0x80000000)) | (r << np.uint32(4))
+    return (hi >> np.uint32(16)).astype(np.uint16), r
 
 
 def f16c_value(codes: np.ndarray) -> np.ndarray:

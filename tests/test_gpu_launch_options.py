@@ -132,3 +132,44 @@ def test_box_steps_same_bits_under_wide_workgroups(hip, precision):
             continue
         for i, (x, y) in enumerate(zip(got, want)):
             assert x.tobytes() == y.tobytes(), (opts, i)
+
+
+def test_pattern_probes_keep_or_reset_the_state(hip):
+    """lbm_probe_pattern (the bandwidth probes of bench.py --workload
+    ceiling): the read-modify-write and read patterns leave every population
+    as it was, the write pattern stores the rest state; bad arguments and
+    non-FP32 solvers are rejected."""
+    import ctypes
+    from cavityflow_cuda_amd import LbmError
+    from cavityflow_cuda_amd._lib import check, lib
+    print("[case] pattern probes", flush=True)
+    s, full = state(hip, "f32", np.random.default_rng(3))
+    before = s.getDensityDistributions(*full).copy()
+    nx, ny, _ = s.size_h
+    pitch = (nx + 31) // 32 * 32
+    span = {0: 0, 1: nx * ny + nx, 2: 0, 4: pitch * ny + pitch, 5: 1, 6: nx * ny + nx}
+    for mode in (0, 1, 2, 4, 5, 6):
+        for opt in (0, 1, 16, 17, 8, 4, 256, 512, 512 | 17, 256 | 4):
+            n = ctypes.c_uint64()
+            check(lib().lbm_probe_pattern(s.handle, mode, opt, ctypes.byref(n), None))
+            s.sync()
+            per = 1 if mode == 2 else (4 if opt & 512 else (2 if opt & 256 else 1))
+            cells = int(np.prod(s.size_h)) - 2 * span[mode]
+            assert n.value == cells // per * per, (mode, opt)
+            assert s.getDensityDistributions(*full).tobytes() == before.tobytes(), (mode, opt)
+    check(lib().lbm_probe_pattern(s.handle, 3, 1, None, None))
+    s.sync()
+    w = np.array([1 / 18] * 4 + [1 / 36] * 12 + [1 / 18] * 2 + [1 / 3], np.float32)
+    got = s.getDensityDistributions(*full).reshape(19, -1)
+    assert np.array_equal(got, np.repeat(w[:, None], got.shape[1], axis=1))
+    with pytest.raises(LbmError):
+        check(lib().lbm_probe_pattern(s.handle, 7, 0, None, None))
+    with pytest.raises(LbmError):
+        check(lib().lbm_probe_pattern(s.handle, 0, 2, None, None))
+    with pytest.raises(LbmError):
+        check(lib().lbm_probe_pattern(s.handle, 0, 1024, None, None))
+    s.close()
+    d = hip.LbmSolverHIP((8, 8, 8), precision="f64")
+    with pytest.raises(LbmError):
+        check(lib().lbm_probe_pattern(d.handle, 0, 0, None, None))
+    d.close()
