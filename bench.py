@@ -686,6 +686,53 @@ def ceiling_workload(args, cf, L, hip, stream):
     print(json.dumps(line), flush=True)
 
 
+def split_workload(args, cf, L, hip, stream):
+    """The cavity divided into subdomains on the one GPU (CavityController with
+    `split`), faces exchanged on the device after every half-step: the serial
+    order (whole subdomains, then the exchange) against the reference's
+    boundary-first order with the exchange on a second stream
+    (overlap=True); both give the same bits."""
+    from cavityflow_cuda_amd import _hip
+    size = (args.size,) * 3
+    split = tuple(args.split)
+    params = cf.cavity_parameters(size, args.u_lid, args.reynolds, args.precision)
+    cells = int(np.prod(size))
+    runs = {}
+    for overlap in (False, True):
+        main_stream = _hip.Stream(0)
+        c = cf.CavityController(size, precision=args.precision, store_outputs=False,
+                                split=split, params=params, stream=main_stream.handle,
+                                overlap=overlap)
+        st = ctypes.c_void_p(main_stream.handle)
+        for _ in range(args.warmup):
+            c.computeNextStep()
+        hip.sync(st)
+        e0, e1 = hip.event(), hip.event()
+        hip.record(e0, st)
+        for _ in range(args.steps):
+            c.computeNextStep()
+        hip.record(e1, st)
+        ms = hip.elapsed_ms(e0, e1)
+        key = "overlap" if overlap else "serial"
+        runs[key] = dict(mlups=round(cells * args.steps / (ms * 1e-3) / 1e6, 1),
+                         ms_per_step=round(ms / args.steps, 4))
+        print(f"[split] {key} {runs[key]}", file=sys.stderr, flush=True)
+        c.close()
+        main_stream.close()
+    line = {
+        "metric": "MLUPS, cavity split into subdomains on one GPU",
+        "value": runs["overlap"]["mlups"], "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "higher_is_better": True, "dtype": args.precision,
+        "data": "synthetic",
+        "config": {"workload": f"lid-driven cavity {size[0]}^3, split {split[0]}x{split[1]}x"
+                               f"{split[2]}, device-side face exchange every half-step",
+                   "reynolds": args.reynolds, "u_lid": args.u_lid},
+        "runs": runs,
+        "overlap_over_serial": round(runs["overlap"]["mlups"] / runs["serial"]["mlups"], 4),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -718,7 +765,9 @@ def main():
                     help="LBM_LAUNCH_* bits for the alpha kernel (default: library default)")
     ap.add_argument("--launch-beta", type=int, default=None)
     ap.add_argument("--workload", default="step",
-                    choices=("step", "precision", "batched", "serving", "ceiling"))
+                    choices=("step", "precision", "batched", "serving", "ceiling", "split"))
+    ap.add_argument("--split", type=int, nargs=3, default=(2, 2, 2),
+                    help="split: subdomains per axis")
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--cavities", type=int, default=64, help="batched: number of cavities")
     ap.add_argument("--no-graph", action="store_true",
@@ -764,6 +813,11 @@ def main():
         return
     if args.workload == "ceiling":
         ceiling_workload(args, cf, L, hip, stream)
+        return
+    if args.workload == "split":
+        if args.steps == ap.get_default("steps"):
+            args.steps = 200
+        split_workload(args, cf, L, hip, stream)
         return
     run = step_run(args, cf, L, hip, stream, args.precision, args.steps, args.warmup)
     size, params, mlups, total_ms = run["size"], run["params"], run["mlups"], run["total_ms"]

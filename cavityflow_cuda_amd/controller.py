@@ -15,6 +15,12 @@ surface of the reference, on one GPU.
   faces on the device in the reference order x, then y, then z
   (CController.cpp:369-570 after alpha, 807-1008 after beta), with
   ``lbm_exchange_face`` standing in for the MPI send/recv pair.
+* ``overlap=True`` keeps the reference's step order for that exchange
+  (CController.cpp:209-309, 647-747): each subdomain first steps the two
+  cells next to every communicating face (the halo and send planes), then its
+  inner box, and the faces are exchanged on a second stream while the inner
+  boxes run. The inner boxes touch neither the planes the exchange reads nor
+  the slots it writes, so the result is bit-identical to the serial order.
 * ``run`` times ``loops`` steps and reports the reference's benchmark figures
   (CController.cpp:1382-1490): GLUPS over interior cells and its
   "bandwidth" = GLUPS * (19 * 2 * sizeof(T) + sizeof(Flag) [+ 4 * sizeof(T)]).
@@ -46,7 +52,8 @@ class CavityController:
     """Global cavity of ``size`` interior cells split into ``split`` subdomains."""
 
     def __init__(self, size, u_lid=0.05, reynolds=100.0, precision="f32",
-                 store_outputs=True, split=(1, 1, 1), device=0, params=None, stream=None):
+                 store_outputs=True, split=(1, 1, 1), device=0, params=None, stream=None,
+                 overlap=False):
         self.size = tuple(int(v) for v in size)
         self.split = tuple(int(v) for v in split)
         for a in range(3):
@@ -86,6 +93,23 @@ class CavityController:
             if j == py - 1:
                 s.setDrivenCavitySzenario()
             self.solvers[(i, j, k)] = s
+        self.overlap = bool(overlap) and len(self.solvers) > 1
+        self._streams = []
+        if self.overlap:
+            from ._hip import Event, Stream
+            for a in range(3):
+                # two slabs of two cells must fit without overlapping
+                if self.split[a] > 1 and self.sub[a] < 2:
+                    raise ValueError("overlap needs at least 2 interior cells per subdomain "
+                                     "along every split axis")
+            if self.stream is None:
+                self._streams.append(Stream(device))
+                self.stream = self._streams[-1].handle
+            self._exchange_stream = Stream(device)
+            self._streams.append(self._exchange_stream)
+            self._slabs_done = Event()
+            self._exchange_done = Event()
+            self._boxes = {key: self._slab_boxes(key) for key in self.solvers}
 
     @classmethod
     def from_config(cls, cfg, device=0, stream=None, vis_format="netcdf") -> "CavityController":
@@ -133,14 +157,63 @@ class CavityController:
                 if (p := w.render(iteration, self.solvers[key])) is not None]
 
     def close(self):
+        for st in self._streams:
+            st.close()
+        self._streams = []
+        if self.overlap:
+            self._slabs_done.close()
+            self._exchange_done.close()
+            self.overlap = False
         for s in self.solvers.values():
             s.close()
         self.solvers = {}
 
     # -- stepping ---------------------------------------------------------
-    def _exchange(self, phase: int):
+    def _slab_boxes(self, key):
+        """CController::stepAlpha's boxes (CController.cpp:209-249): for each
+        face with a neighbour, in the order LEFT, RIGHT, BOTTOM, TOP, BACK,
+        FRONT, the two cells next to it over the current inner box's extent
+        of the other axes, the inner box shrinking by those two cells.
+        Returns (slabs, inner or None when nothing is left inside)."""
+        s = self.solvers[key]
+        org, ext = [0, 0, 0], list(s.size_h)
+        slabs = []
+        for d in range(6):
+            axis, hi = d // 2, d & 1
+            nb = list(key)
+            nb[axis] += 1 if hi else -1
+            if tuple(nb) not in self.solvers:
+                continue
+            bo, bz = list(org), list(ext)
+            bz[axis] = 2
+            if hi:
+                bo[axis] = s.size_h[axis] - 2
+            else:
+                org[axis] += 2
+            ext[axis] -= 2
+            slabs.append((tuple(bo), tuple(bz)))
+        inner = (tuple(org), tuple(ext)) if min(ext) > 0 else None
+        return slabs, inner
+
+    def _step_overlapped(self, phase: int):
+        step = "simulationStepBeta" if phase else "simulationStepAlpha"
+        for key, s in self.solvers.items():
+            for box in self._boxes[key][0]:
+                getattr(s, step)(*box, stream=self.stream)
+        self._slabs_done.record(self.stream)
+        self._slabs_done.wait_on(self._exchange_stream.handle)
+        self._exchange(phase, self._exchange_stream.handle)
+        self._exchange_done.record(self._exchange_stream.handle)
+        for key, s in self.solvers.items():
+            inner = self._boxes[key][1]
+            if inner is not None:
+                getattr(s, step)(*inner, stream=self.stream)
+        # the next half-step reads what the exchange wrote
+        self._exchange_done.wait_on(self.stream)
+
+    def _exchange(self, phase: int, stream=None):
         L = lib()
-        st = _stream(self.stream)
+        st = _stream(self.stream if stream is None else stream)
         for axis in range(3):
             lo_dir, hi_dir = 2 * axis, 2 * axis + 1
             for key, a in self.solvers.items():
@@ -153,12 +226,16 @@ class CavityController:
                 check(L.lbm_exchange_face(b.handle, lo_dir, phase, a.handle, st))
 
     def stepAlpha(self):
+        if self.overlap:
+            return self._step_overlapped(0)
         for s in self.solvers.values():
             s.simulationStepAlpha(stream=self.stream)
         if len(self.solvers) > 1:
             self._exchange(0)
 
     def stepBeta(self):
+        if self.overlap:
+            return self._step_overlapped(1)
         for s in self.solvers.values():
             s.simulationStepBeta(stream=self.stream)
         if len(self.solvers) > 1:
@@ -199,6 +276,7 @@ class CavityController:
 
     # -- extraction -----------------------------------------------------------
     def _gather(self, getter, ncomp, dtype=None):
+        self.sync()   # the getters copy on the default stream
         out = np.empty((ncomp,) + self.size[::-1], dtype=dtype or self.dtype)
         sx, sy, sz = self.sub
         for (i, j, k), s in self.solvers.items():

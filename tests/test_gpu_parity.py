@@ -160,6 +160,37 @@ def test_split_equals_monolithic(hip, split):
     parts.close()
 
 
+@pytest.mark.parametrize("split,size", [((2, 1, 1), (24, 20, 18)), ((1, 2, 1), (16, 24, 20)),
+                                        ((1, 1, 2), (18, 16, 24)), ((2, 2, 2), (24, 24, 24)),
+                                        ((3, 1, 2), (30, 14, 16)), ((2, 2, 1), (4, 4, 9))])
+@pytest.mark.parametrize("prec", ["f32", "f16s"])
+def test_overlapped_exchange_equals_serial_order(hip, split, size, prec):
+    """CController's boundary-first order (CController.cpp:209-309, 647-747):
+    slabs next to the communicating faces, then the inner boxes while the
+    faces are exchanged on a second stream. Bit-identical to whole-subdomain
+    steps followed by the exchange, and to the undivided cavity; the smallest
+    case has subdomains of 2 cells, so slabs fill them and no inner box is
+    left."""
+    say("overlapped exchange", split, size, prec)
+    # Re scaled with the edge so tau stays inside the reference's limits
+    p = hip.cavity_parameters(size, 0.08, 10.0 * max(size), prec)
+    runs = {}
+    for name, kw in (("mono", dict()), ("serial", dict(split=split)),
+                     ("overlap", dict(split=split, overlap=True))):
+        c = hip.CavityController(size, precision=prec, params=p, **kw)
+        assert c.overlap == (name == "overlap")
+        for _ in range(41):
+            c.computeNextStep()
+        runs[name] = (c.getVelocities().tobytes(), c.getDensities().tobytes())
+        c.close()
+    assert runs["overlap"] == runs["serial"] == runs["mono"]
+
+
+def test_overlap_rejects_subdomains_too_thin(hip):
+    with pytest.raises(ValueError):
+        hip.CavityController((2, 8, 8), split=(2, 1, 1), overlap=True)
+
+
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_box_copies_match_oracle(hip, oracle_cpu, dtype):
     say("box copies", dtype.__name__)
