@@ -53,7 +53,7 @@ class CavityController:
 
     def __init__(self, size, u_lid=0.05, reynolds=100.0, precision="f32",
                  store_outputs=True, split=(1, 1, 1), device=0, params=None, stream=None,
-                 overlap=False):
+                 overlap=False, xslab=2):
         self.size = tuple(int(v) for v in size)
         self.split = tuple(int(v) for v in split)
         for a in range(3):
@@ -94,6 +94,9 @@ class CavityController:
                 s.setDrivenCavitySzenario()
             self.solvers[(i, j, k)] = s
         self.overlap = bool(overlap) and len(self.solvers) > 1
+        # x slabs may be thicker than the reference's two cells: a slab two cells
+        # wide touches one cache line per row and plane for 8 bytes of data
+        self.xslab = max(2, int(xslab))
         self._streams = []
         if self.overlap:
             from ._hip import Event, Stream
@@ -177,20 +180,23 @@ class CavityController:
         Returns (slabs, inner or None when nothing is left inside)."""
         s = self.solvers[key]
         org, ext = [0, 0, 0], list(s.size_h)
+        faces = [d for d in range(6)
+                 if tuple(v + ((1 if d & 1 else -1) if a == d // 2 else 0)
+                          for a, v in enumerate(key)) in self.solvers]
+        nx_faces = sum(1 for d in faces if d < 2)
+        # the x slab width, as long as the slabs do not meet
+        wx = self.xslab if nx_faces * self.xslab <= s.size_h[0] else 2
         slabs = []
-        for d in range(6):
+        for d in faces:
             axis, hi = d // 2, d & 1
-            nb = list(key)
-            nb[axis] += 1 if hi else -1
-            if tuple(nb) not in self.solvers:
-                continue
+            w = wx if axis == 0 else 2
             bo, bz = list(org), list(ext)
-            bz[axis] = 2
+            bz[axis] = w
             if hi:
-                bo[axis] = s.size_h[axis] - 2
+                bo[axis] = s.size_h[axis] - w
             else:
-                org[axis] += 2
-            ext[axis] -= 2
+                org[axis] += w
+            ext[axis] -= w
             slabs.append((tuple(bo), tuple(bz)))
         inner = (tuple(org), tuple(ext)) if min(ext) > 0 else None
         return slabs, inner

@@ -176,14 +176,15 @@ def test_overlapped_exchange_equals_serial_order(hip, split, size, prec):
     p = hip.cavity_parameters(size, 0.08, 10.0 * max(size), prec)
     runs = {}
     for name, kw in (("mono", dict()), ("serial", dict(split=split)),
-                     ("overlap", dict(split=split, overlap=True))):
+                     ("overlap", dict(split=split, overlap=True)),
+                     ("overlap_x8", dict(split=split, overlap=True, xslab=8))):
         c = hip.CavityController(size, precision=prec, params=p, **kw)
-        assert c.overlap == (name == "overlap")
+        assert c.overlap == name.startswith("overlap")
         for _ in range(41):
             c.computeNextStep()
         runs[name] = (c.getVelocities().tobytes(), c.getDensities().tobytes())
         c.close()
-    assert runs["overlap"] == runs["serial"] == runs["mono"]
+    assert runs["overlap"] == runs["overlap_x8"] == runs["serial"] == runs["mono"]
 
 
 def test_overlap_rejects_subdomains_too_thin(hip):
