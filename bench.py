@@ -484,9 +484,9 @@ def step_run(args, cf, L, hip, stream, precision, steps, warmup):
         check(L.lbm_get_launch_options(solver.handle, k, ctypes.byref(v)))
         launch_opts.append(v.value)
 
-    # Measured at 512^3: one call for both x faces halves the cache lines the
-    # halo-plane unpack dirties, which pays for 16-bit storage; with FP32 the
-    # following beta absorbs the deferred write-back and loses what it saves.
+    # x-face unpack: one paired call for 16-bit storage, one call per face for
+    # FP32 / F64; with 5 planes per face the two measure the same at 512^3
+    # (FP32 33,056 vs 33,084 MLUPS, F16S / F16C within 0.5 %).
     separate_unpack = (args.unpack == "separate" or
                        (args.unpack == "auto" and precision in ("f32", "f64")))
 
@@ -590,18 +590,31 @@ def step_run(args, cf, L, hip, stream, precision, steps, warmup):
                 wall=wall, halo_ms=halo_ms, step_ms_total=step_ms_total, steps=steps)
 
 
+PROBE_OPTION_NAMES = ((1, "NT"), (4, "WAVES4"), (8, "WAVES3"), (16, "WAVES2"), (256, "VEC2"),
+                      (512, "VEC4"), (1024, "B512"), (2048, "B1024"), (4096, "SPAN2"),
+                      (8192, "SPAN4"), (16384, "SPAN8"))
+
+
+def probe_option_name(opt):
+    return "+".join(n for bit, n in PROBE_OPTION_NAMES if opt & bit) or "none"
+
+
 def ceiling_workload(args, cf, L, hip, stream):
     """Bandwidth ceilings of the step kernels' access patterns at the bench
-    size (FP32): the library's pattern probes (lbm_probe_pattern: alpha's
-    aligned 19-plane read-modify-write, beta's shifted one, 19-plane read and
-    write streams) under the launch options the steps use, a device-to-device
-    copy, and the step kernels themselves (no exchange) for comparison. Each
-    figure is bytes moved / mean launch time over `reps` launches timed with
-    HIP events on the launch stream."""
+    size and storage precision (`--precision`, FP32 by default): the library's
+    pattern probes (lbm_probe_pattern: alpha's aligned 19-plane
+    read-modify-write, beta's shifted one, 19-plane read and write streams)
+    under the launch options the steps use and under workgroup shapes of
+    256-1024 threads and 1-8 cells per thread, a device-to-device copy, and
+    the step kernels themselves (no exchange) for comparison. Each figure is
+    bytes moved / mean launch time over `reps` launches timed with HIP events
+    on the launch stream."""
     from cavityflow_cuda_amd._lib import check
     size = (args.size,) * 3
+    prec = args.precision
+    bpp = BYTES_PER_POP[prec]
     reps = 10
-    s = cf.LbmSolverHIP(size, (cf.OBSTACLE,) * 6, "f32", inv_tau=1.0 / 0.6536,
+    s = cf.LbmSolverHIP(size, (cf.OBSTACLE,) * 6, prec, inv_tau=1.0 / 0.6536,
                         lid_velocity=args.u_lid, store_densities=False, store_velocities=False)
     s.setDrivenCavitySzenario()
     e0, e1 = hip.event(), hip.event()
@@ -618,17 +631,29 @@ def ceiling_workload(args, cf, L, hip, stream):
     names = {0: "rmw_aligned", 1: "rmw_shifted", 2: "read", 3: "write",
              4: "rmw_shifted_pitched", 5: "rmw_shifted_x", 6: "rmw_shifted_yz"}
     vec = [256, 512, 256 | 16, 512 | 16, 256 | 8, 512 | 8, 256 | 4, 512 | 4, 512 | 1 | 16]
-    opt_sets = {0: [0, 1, 16, 17, 8, 9, 4] + vec, 1: [0, 1, 16, 17, 8, 9, 4] + vec,
-                2: [0, 1, 16], 3: [0, 1, 16], 4: [0, 16, 8, 4], 5: [0, 16, 8, 4],
-                6: [0, 16, 8, 4]}
+    # workgroup shapes: 256 / 512 / 1024 threads x 1 / 2 / 4 / 8 cells per
+    # thread, uncapped or capped at 4 or 2 waves per SIMD (where one
+    # workgroup fits), cached and nontemporal
+    shapes = []
+    for b in (0, 1024, 2048):
+        for sp in (0, 4096, 8192, 16384):
+            for cap in (0, 4, 16):
+                if cap == 16 and b == 2048:
+                    continue
+                shapes += [b | sp | cap, b | sp | cap | 1]
+    base = [0, 1, 16, 17, 8, 9, 4] + (vec if prec == "f32" else [])
+    rmw = list(dict.fromkeys(base + shapes))
+    opt_sets = {0: rmw, 1: rmw, 2: [0, 1, 16], 3: [0, 1, 16], 4: [0, 16, 8, 4],
+                5: [0, 16, 8, 4], 6: [0, 16, 8, 4]}
     probes = []
     for mode in (2, 3, 0, 1, 4, 5, 6):
         for opt in opt_sets[mode]:
             cells = ctypes.c_uint64()
             check(L.lbm_probe_pattern(s.handle, mode, opt, ctypes.byref(cells), stream))
             ms = timed(lambda: check(L.lbm_probe_pattern(s.handle, mode, opt, None, stream)))
-            nbytes = cells.value * 19 * 4 * (1 if mode in (2, 3) else 2)
-            probes.append(dict(pattern=names[mode], options=opt, ms=round(ms, 4),
+            nbytes = cells.value * 19 * bpp * (1 if mode in (2, 3) else 2)
+            probes.append(dict(pattern=names[mode], options=opt,
+                               option_names=probe_option_name(opt), ms=round(ms, 4),
                                gbs=round(nbytes / (ms * 1e-3) / 1e9, 1)))
             print(f"[ceiling] {probes[-1]}", file=sys.stderr, flush=True)
     # the step kernels (whole domain, library default options, no exchange)
@@ -656,7 +681,7 @@ def ceiling_workload(args, cf, L, hip, stream):
                 check(L.lbm_simulation_step_beta(s.handle, None, None, stream))
                 hip.record(b, stream)
         ms = float(np.mean([hip.elapsed_ms(a, b) for a, b in ev]))
-        nbytes = int(np.prod(size)) * 2 * 19 * 4
+        nbytes = int(np.prod(size)) * 2 * 19 * bpp
         steps["k_alpha" if k == 0 else "k_beta"] = dict(options=v.value, ms=round(ms, 4),
                                                       gbs=round(nbytes / (ms * 1e-3) / 1e9, 1))
     print(f"[ceiling] steps {steps}", file=sys.stderr, flush=True)
@@ -669,9 +694,9 @@ def ceiling_workload(args, cf, L, hip, stream):
     line = {
         "metric": "HBM ceilings of the step kernels' access patterns",
         "value": best["rmw_shifted"]["gbs"], "unit": "GB/s", "n_gpus": 1,
-        "higher_is_better": True, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"{size[0]}^3 interior ({size[0] + 2}^3 with walls) FP32 "
-                               "population array, 19 planes, one thread per cell",
+        "higher_is_better": True, "dtype": prec, "data": "synthetic",
+        "config": {"workload": f"{size[0]}^3 interior ({size[0] + 2}^3 with walls) {prec} "
+                               f"population array, 19 planes of {bpp}-byte elements",
                    "reps": reps},
         "peak_gbs": HBM_PEAK_GBS,
         "device_copy_gbs": copy,

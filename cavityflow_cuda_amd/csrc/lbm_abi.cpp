@@ -119,13 +119,13 @@ int copy_get(lbm_solver* s, int field, const Box& b, int ncomp, void* dst, int d
 {
     const size_t bytes = (size_t)b.cells() * ncomp * s->out_bytes;
     if (dst_on_device) {
-        LBM_HIP(launch_box_get(s, field, b, dst, st), "box get kernel");
+        LBM_HIP(launch_box_get(s, field, b, dst, 0x7FFFFu, st), "box get kernel");
         return LBM_OK;
     }
     int rc = ensure_staging(s, bytes);
     if (rc)
         return rc;
-    LBM_HIP(launch_box_get(s, field, b, s->staging, st), "box get kernel");
+    LBM_HIP(launch_box_get(s, field, b, s->staging, 0x7FFFFu, st), "box get kernel");
     LBM_HIP(hipMemcpyAsync(dst, s->staging, bytes, hipMemcpyDeviceToHost, st), "D2H copy");
     LBM_HIP(hipStreamSynchronize(st), "hipStreamSynchronize");
     return LBM_OK;
@@ -167,6 +167,23 @@ uint32_t direction_mask(int direction)
         if (norm[0] * cx(l) + norm[1] * cy(l) + norm[2] * cz(l) > 0)
             m |= 1u << l;
     return m;
+}
+
+// The lattice vectors a face exchange moves (lbm_pack_face / lbm_unpack_face).
+// `face` is the sending solver's face. In the beta step the slot of plane l at
+// cell p is read and written only by cell p - c_l, so of a halo plane only the
+// 5 planes whose vectors point out through its face (read by the interior
+// cells next to it) reach the interior; the other 14 stay in the halo or are
+// moved (by the linear wrap, or by the halo cells' own pass-through) into
+// interior slots that the phase-1 unpack overwrites before anything reads
+// them. Phase 0 therefore sends those 5 planes of the send plane (vectors
+// pointing back into the sender: direction_mask(face)), phase 1 the 5 planes
+// of the halo plane the receiver's direction filter applies
+// (direction_mask(face ^ 1)). The reference moves all 19 planes after alpha
+// (CController.cpp:369-570); the interiors are the same bit for bit.
+uint32_t exchange_mask(int face, int phase)
+{
+    return direction_mask(phase == 0 ? face : face ^ 1);
 }
 
 // The CLbmSolver constructor's time-step adaptation (CLbmSolver.cpp:75-281)
@@ -717,7 +734,8 @@ int lbm_pack_face(lbm_solver* s, int direction, int phase, void* dev_buf, hipStr
     if ((rc = set_device(s)))
         return rc;
     Box b{o[0], o[1], o[2], z[0], z[1], z[2]};
-    LBM_HIP(launch_box_get(s, 0, b, dev_buf, stream), "pack kernel");
+    LBM_HIP(launch_box_get(s, 0, b, dev_buf, exchange_mask(direction, phase), stream),
+            "pack kernel");
     return LBM_OK;
 }
 
@@ -735,8 +753,9 @@ int lbm_unpack_face(lbm_solver* s, int direction, int phase, const void* dev_buf
     if ((rc = set_device(s)))
         return rc;
     Box b{o[0], o[1], o[2], z[0], z[1], z[2]};
-    const uint32_t mask = phase == 0 ? 0x7FFFFu : direction_mask(direction);
-    LBM_HIP(launch_box_set(s, 0, b, dev_buf, mask, stream), "unpack kernel");
+    // `direction` is the receiving face: the sender's is the opposite one
+    LBM_HIP(launch_box_set(s, 0, b, dev_buf, exchange_mask(direction ^ 1, phase), stream),
+            "unpack kernel");
     return LBM_OK;
 }
 
@@ -758,12 +777,11 @@ int lbm_unpack_face_pair(lbm_solver* s, int axis, int phase, const void* lo_buf,
     // the boxes of lbm_face_box for LEFT / RIGHT
     const int nx = s->size_h[0];
     const int x_lo = phase == 0 ? 0 : 1, x_hi = phase == 0 ? nx - 1 : nx - 2;
-    const uint32_t mask_lo = phase == 0 ? 0x7FFFFu : direction_mask(LBM_LEFT);
-    const uint32_t mask_hi = phase == 0 ? 0x7FFFFu : direction_mask(LBM_RIGHT);
-    // the halo planes (phase 0) share cache lines between the faces: cached
-    // stores merge them; the send planes (phase 1) share none: streaming stores
-    LBM_HIP(launch_set_dd_xpair(s, lo_buf, hi_buf, x_lo, x_hi, mask_lo, mask_hi, phase == 1,
-                                stream),
+    // the LEFT face receives from a RIGHT face, and vice versa; the two faces'
+    // planes are disjoint, so their stores share no cache line: streaming
+    const uint32_t mask_lo = exchange_mask(LBM_RIGHT, phase);
+    const uint32_t mask_hi = exchange_mask(LBM_LEFT, phase);
+    LBM_HIP(launch_set_dd_xpair(s, lo_buf, hi_buf, x_lo, x_hi, mask_lo, mask_hi, true, stream),
             "pair unpack kernel");
     return LBM_OK;
 }

@@ -17,8 +17,11 @@ struct BoxArgs {
     int32_t ox, oy, oz;
     uint32_t cells;
     FastDiv div_bx, div_by;
-    uint32_t mask;
     uint32_t* status;   // raised by a population store out of the storage range
+    // populations: the lattice vectors copied, in order (grid.y = nplanes);
+    // buffers keep all 19 planes, [l][cell], the others left alone
+    uint8_t plane[kQ];
+    int nplanes;
 };
 
 __device__ __forceinline__ uint32_t box_index(const BoxArgs& a, uint32_t gid)
@@ -30,15 +33,16 @@ __device__ __forceinline__ uint32_t box_index(const BoxArgs& a, uint32_t gid)
     return (uint32_t)((a.oz + (int32_t)z) * a.nxy + (a.oy + (int32_t)y) * a.nx + a.ox + (int32_t)x);
 }
 
-// populations: grid.y = lattice vector; shifted storage is converted to f
-// NT: streaming accesses, for boxes one cell thick on some axis (the faces of
-// a halo exchange), which the next step does not reuse from the caches.
+// populations: grid.y = index into the plane list; shifted storage is
+// converted to f. NT: streaming accesses, for boxes one cell thick on some axis
+// (the faces of a halo exchange), which the next step does not reuse from the
+// caches.
 template <class P, bool NT>
 __global__ __launch_bounds__(kCopyBlock) void k_get_dd(const typename P::S* __restrict__ dd,
                                                        typename P::C* __restrict__ dst,
                                                        BoxArgs a)
 {
-    const int l = blockIdx.y;
+    const int l = a.plane[blockIdx.y];
     const uint32_t gid = blockIdx.x * kCopyBlock + threadIdx.x;
     if (gid >= a.cells)
         return;
@@ -53,9 +57,7 @@ __global__ __launch_bounds__(kCopyBlock) void k_set_dd(typename P::S* __restrict
                                                        const typename P::C* __restrict__ src,
                                                        BoxArgs a)
 {
-    const int l = blockIdx.y;
-    if (!((a.mask >> l) & 1u))
-        return;
+    const int l = a.plane[blockIdx.y];
     const uint32_t gid = blockIdx.x * kCopyBlock + threadIdx.x;
     if (gid >= a.cells)
         return;
@@ -67,6 +69,21 @@ __global__ __launch_bounds__(kCopyBlock) void k_set_dd(typename P::S* __restrict
         __builtin_nontemporal_store(v, dd + (int64_t)l * a.stride + id);
     else
         dd[(int64_t)l * a.stride + id] = v;
+}
+
+// The lattice vectors of a component mask, in order.
+struct PlaneList {
+    uint8_t plane[kQ];
+    int n;
+};
+
+static PlaneList plane_list(uint32_t mask)
+{
+    PlaneList p{};
+    for (int l = 0; l < kQ; l++)
+        if ((mask >> l) & 1u)
+            p.plane[p.n++] = (uint8_t)l;
+    return p;
 }
 
 // Both x faces of a solver in one launch: thread (l, r), r = 0..rows, writes
@@ -82,12 +99,11 @@ __global__ __launch_bounds__(kCopyBlock) void k_set_dd_xpair(typename P::S* __re
                                                              int64_t stride, int32_t nx,
                                                              int32_t x_lo, int32_t x_hi,
                                                              uint32_t rows, uint32_t mask_lo,
-                                                             uint32_t mask_hi, uint32_t* status)
+                                                             uint32_t mask_hi, uint32_t* status,
+                                                             PlaneList planes)
 {
-    const int l = blockIdx.y;
+    const int l = planes.plane[blockIdx.y];
     const bool on_lo = (mask_lo >> l) & 1u, on_hi = (mask_hi >> l) & 1u;
-    if (!on_lo && !on_hi)
-        return;
     const uint32_t r = blockIdx.x * kCopyBlock + threadIdx.x;
     if (r > rows)
         return;
@@ -160,16 +176,22 @@ static BoxArgs box_args(const lbm_solver* s, const Box& b, uint32_t mask)
     a.cells = b.cells();
     a.div_bx = FastDiv((uint32_t)b.bx);
     a.div_by = FastDiv((uint32_t)b.by);
-    a.mask = mask;
     a.status = s->status;
+    const PlaneList p = plane_list(mask);
+    for (int l = 0; l < kQ; l++)
+        a.plane[l] = p.plane[l];
+    a.nplanes = p.n;
     return a;
 }
 
 template <class P>
-static hipError_t get_dd_t(const lbm_solver* s, const Box& b, void* dst, hipStream_t st)
+static hipError_t get_dd_t(const lbm_solver* s, const Box& b, void* dst, uint32_t mask,
+                           hipStream_t st)
 {
-    BoxArgs a = box_args(s, b, 0x7FFFFu);
-    dim3 grid((a.cells + kCopyBlock - 1) / kCopyBlock, kQ);
+    BoxArgs a = box_args(s, b, mask);
+    if (!a.nplanes)
+        return hipSuccess;
+    dim3 grid((a.cells + kCopyBlock - 1) / kCopyBlock, a.nplanes);
     if (b.bx == 1 || b.by == 1 || b.bz == 1)
         hipLaunchKernelGGL((k_get_dd<P, true>), grid, dim3(kCopyBlock), 0, st,
                            (const typename P::S*)s->dd, (typename P::C*)dst, a);
@@ -184,7 +206,9 @@ static hipError_t set_dd_t(const lbm_solver* s, const Box& b, const void* src, u
                            hipStream_t st)
 {
     BoxArgs a = box_args(s, b, mask);
-    dim3 grid((a.cells + kCopyBlock - 1) / kCopyBlock, kQ);
+    if (!a.nplanes)
+        return hipSuccess;
+    dim3 grid((a.cells + kCopyBlock - 1) / kCopyBlock, a.nplanes);
     if (b.bx == 1 || b.by == 1 || b.bz == 1)
         hipLaunchKernelGGL((k_set_dd<P, true>), grid, dim3(kCopyBlock), 0, st,
                            (typename P::S*)s->dd, (const typename P::C*)src, a);
@@ -210,15 +234,15 @@ static hipError_t field_t(const lbm_solver* s, int field, const Box& b, void* bu
 }
 
 hipError_t launch_box_get(const lbm_solver* s, int field, const Box& b, void* dst,
-                          hipStream_t st)
+                          uint32_t mask, hipStream_t st)
 {
     if (field == 0) {
         switch (s->p.precision) {
-        case LBM_F32: return get_dd_t<StoreF32>(s, b, dst, st);
-        case LBM_F64: return get_dd_t<StoreF64>(s, b, dst, st);
-        case LBM_F16S: return get_dd_t<StoreF16S>(s, b, dst, st);
-        case LBM_BF16S: return get_dd_t<StoreBF16S>(s, b, dst, st);
-        case LBM_F16C: return get_dd_t<StoreF16C>(s, b, dst, st);
+        case LBM_F32: return get_dd_t<StoreF32>(s, b, dst, mask, st);
+        case LBM_F64: return get_dd_t<StoreF64>(s, b, dst, mask, st);
+        case LBM_F16S: return get_dd_t<StoreF16S>(s, b, dst, mask, st);
+        case LBM_BF16S: return get_dd_t<StoreBF16S>(s, b, dst, mask, st);
+        case LBM_F16C: return get_dd_t<StoreF16C>(s, b, dst, mask, st);
         default: return hipErrorInvalidValue;
         }
     }
@@ -251,17 +275,20 @@ static hipError_t set_dd_xpair_t(const lbm_solver* s, const void* lo, const void
                                  hipStream_t st)
 {
     const uint32_t rows = (uint32_t)s->size_h[1] * (uint32_t)s->size_h[2];
-    dim3 grid((rows + 1 + kCopyBlock - 1) / kCopyBlock, kQ);
+    const PlaneList planes = plane_list(mask_lo | mask_hi);
+    if (!planes.n)
+        return hipSuccess;
+    dim3 grid((rows + 1 + kCopyBlock - 1) / kCopyBlock, planes.n);
     if (nt)
         hipLaunchKernelGGL((k_set_dd_xpair<P, true>), grid, dim3(kCopyBlock), 0, st,
                            (typename P::S*)s->dd, (const typename P::C*)lo,
                            (const typename P::C*)hi, s->stride, s->size_h[0], x_lo, x_hi, rows,
-                           mask_lo, mask_hi, s->status);
+                           mask_lo, mask_hi, s->status, planes);
     else
         hipLaunchKernelGGL((k_set_dd_xpair<P, false>), grid, dim3(kCopyBlock), 0, st,
                            (typename P::S*)s->dd, (const typename P::C*)lo,
                            (const typename P::C*)hi, s->stride, s->size_h[0], x_lo, x_hi, rows,
-                           mask_lo, mask_hi, s->status);
+                           mask_lo, mask_hi, s->status, planes);
     return hipGetLastError();
 }
 

@@ -115,21 +115,21 @@ __device__ __forceinline__ bool xpack_where(const StepArgs<C>& a, uint32_t id, u
     return pl || pr;
 }
 
-// After alpha: the 19 slot values of a send-plane cell, in the layout
-// lbm_pack_face writes ([slot][row]); values pass through the storage type so
-// 16-bit storage packs exactly what was stored.
+// After alpha: the slot values of a send-plane cell that the neighbour's halo
+// needs, the planes whose vectors point back into this solver (c_x = +1 on the
+// LEFT send plane, c_x = -1 on the RIGHT one), in the layout lbm_pack_face
+// writes ([slot][row]); values pass through the storage type so 16-bit storage
+// packs exactly what was stored.
 template <class P>
 __device__ __forceinline__ void xpack_alpha(const StepArgs<typename P::C>& a, uint32_t row,
                                             bool pl, bool pr, const typename P::C (&v)[kQ])
 {
     using C = typename P::C;
 #pragma unroll
-    for (int s = 0; s < kQ; s++) {
-        const C w = P::load(P::store(v[s], s), s);
-        if (pl)
-            a.xpack[0][(int64_t)s * a.fcells + row] = w;
-        if (pr)
-            a.xpack[1][(int64_t)s * a.fcells + row] = w;
+    for (int s = 0; s < kQ - 1; s++) {
+        if (cx(s) == 0 || !(cx(s) > 0 ? pl : pr))
+            continue;
+        a.xpack[cx(s) > 0 ? 0 : 1][(int64_t)s * a.fcells + row] = P::load(P::store(v[s], s), s);
     }
 }
 

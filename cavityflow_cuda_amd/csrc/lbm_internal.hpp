@@ -103,9 +103,10 @@ hipError_t launch_init(const lbm_solver* s, hipStream_t st, int write_flags);
 hipError_t launch_alpha(const lbm_solver* s, const Box& b, hipStream_t st);
 hipError_t launch_beta(const lbm_solver* s, const Box& b, hipStream_t st);
 
-// field: 0 populations (19 components, masked), 1 velocities (3), 2 densities (1)
+// field: 0 populations (the lattice vectors of `mask`; buffers keep all 19
+// planes), 1 velocities (3), 2 densities (1)
 hipError_t launch_box_get(const lbm_solver* s, int field, const Box& b, void* dst,
-                          hipStream_t st);
+                          uint32_t mask, hipStream_t st);
 // both x faces in one launch (nt: streaming stores)
 hipError_t launch_set_dd_xpair(const lbm_solver* s, const void* lo, const void* hi, int x_lo,
                                int x_hi, uint32_t mask_lo, uint32_t mask_hi, bool nt,

@@ -1,24 +1,28 @@
 // Bandwidth probes of the step kernels' memory access patterns, without the
 // collision: the ceilings that bound k_alpha and k_beta on this memory system
 // (bench.py --workload ceiling). They run on a solver's own population array
-// (19 planes at the solver's plane stride, one thread per cell, 256-thread
-// workgroups) and leave its values unchanged, except the write pattern, which
-// stores the lattice weights (the rest state).
+// (19 planes at the solver's plane stride, elements of its storage size: 4
+// bytes FP32, 8 FP64, 2 for the 16-bit formats) and leave its values
+// unchanged, except the write pattern, which stores the rest state.
 //
 //   RMW_ALIGNED : each cell loads and stores its own 19 slots (alpha).
 //   RMW_SHIFTED : each cell loads and stores plane l at id + c_l (beta), for
 //                 the cells whose 19 neighbours lie inside the array.
 //   RMW_SHIFTED_PITCHED: beta's pattern as a layout with rows padded to a
-//                 multiple of 32 elements (128 bytes) would give it: shifts
+//                 multiple of 32 elements would give it: shifts
 //                 c_x + c_y P + c_z P ny, P = nx rounded up to 32 (y and z
-//                 shifts line-aligned, x shifts by one element).
+//                 shifts line-aligned for FP32, x shifts by one element).
 //   RMW_SHIFTED_X: the x components of the shifts only (one element).
 //   RMW_SHIFTED_YZ: the y and z components only (nx and nx ny elements).
 //   READ        : 19 loads per cell, nothing stored.
 //   WRITE       : 19 stores per cell.
 // Launch options: LBM_LAUNCH_NONTEMPORAL and the WAVES4/3/2 occupancy caps,
-// as for the step kernels; LBM_PROBE_VEC2 / VEC4 give the read-modify-write
-// patterns 2 or 4 consecutive cells per thread, one vector access per plane.
+// as for the step kernels; the workgroup shape of the read-modify-write
+// patterns (LBM_PROBE_BLOCK512 / 1024 threads, LBM_PROBE_SPAN2 / 4 / 8 cells
+// per thread one block apart, as the step kernels' WIDE shape does with 2):
+// each workgroup then covers block x span consecutive cells; or, FP32 only,
+// LBM_PROBE_VEC2 / VEC4: 2 or 4 consecutive cells per thread, one vector
+// access per plane.
 #include "lbm_d3q19.hpp"
 #include "lbm_internal.hpp"
 
@@ -28,13 +32,32 @@ namespace lbm {
 
 namespace {
 
-constexpr int kProbeBlock = 256;
-
 struct ProbeArgs {
     const char* plane[kQ];   // byte base of plane l (RMW_SHIFTED: shifted by c_l)
     uint32_t first, last;    // cells [first, last)
     float zero;              // 0.0f at run time: keeps the stored value a function of the load
+    uint32_t zero_bits;      // 0 at run time, the same for 16-bit codes
+    uint32_t never;          // 0xFFFFFFFF: no sum of 19 16-bit codes reaches it
 };
+
+// The stored value: the loaded one through a run-time identity.
+__device__ inline float keep(float v, const ProbeArgs& a) { return v + a.zero; }
+__device__ inline double keep(double v, const ProbeArgs& a) { return v + (double)a.zero; }
+__device__ inline uint16_t keep(uint16_t v, const ProbeArgs& a)
+{
+    return (uint16_t)(v ^ a.zero_bits);
+}
+
+// The rest state of plane l: the lattice weight, or code 0 (f - w = 0) for
+// the 16-bit formats.
+template <class E>
+__device__ inline E rest(int l)
+{
+    if constexpr (sizeof(E) == 2)
+        return 0;
+    else
+        return weight<E>(l);
+}
 
 // VEC consecutive cells per thread, accessed as one VEC-float vector per
 // plane (4-byte aligned: the shifted planes start at any element).
@@ -50,10 +73,10 @@ struct VecOf<4> {
 };
 
 template <int MODE, bool NT, int VEC>
-__global__ __launch_bounds__(kProbeBlock) void k_probe_vec(ProbeArgs a)
+__global__ __launch_bounds__(256) void k_probe_vec(ProbeArgs a)
 {
     using V = typename VecOf<VEC>::T;
-    const uint32_t id = a.first + (blockIdx.x * kProbeBlock + threadIdx.x) * VEC;
+    const uint32_t id = a.first + (blockIdx.x * 256 + threadIdx.x) * VEC;
     if (id + VEC > a.last)
         return;
     const uint32_t off = id * (uint32_t)sizeof(float);
@@ -78,38 +101,47 @@ __global__ __launch_bounds__(kProbeBlock) void k_probe_vec(ProbeArgs a)
     }
 }
 
-template <int MODE, bool NT>
-__global__ __launch_bounds__(kProbeBlock) void k_probe(ProbeArgs a)
+// One cell: its 19 accesses at the shared byte offset of plane l's base.
+template <class E, int MODE, bool NT>
+__device__ inline void probe_cell(const ProbeArgs& a, uint32_t id)
 {
-    const uint32_t id = a.first + blockIdx.x * kProbeBlock + threadIdx.x;
-    if (id >= a.last)
-        return;
-    const uint32_t off = id * (uint32_t)sizeof(float);
-    float v[kQ];
+    const uint32_t off = id * (uint32_t)sizeof(E);
+    E v[kQ];
     if (MODE != LBM_PROBE_WRITE) {
 #pragma unroll
         for (int l = 0; l < kQ; l++) {
-            const __attribute__((address_space(1))) float* p =
-                (const __attribute__((address_space(1))) float*)(a.plane[l] + off);
+            const __attribute__((address_space(1))) E* p =
+                (const __attribute__((address_space(1))) E*)(a.plane[l] + off);
             v[l] = NT ? __builtin_nontemporal_load(p) : *p;
         }
     }
     if (MODE == LBM_PROBE_READ) {
-        float s = 0.f;
+        // a store that never happens keeps the loads live
+        if constexpr (sizeof(E) == 2) {
+            uint32_t s = 0;
 #pragma unroll
-        for (int l = 0; l < kQ; l++)
-            s += v[l];
-        if (s == 1e30f)   // never: the loads stay live, nothing is stored
-            *(__attribute__((address_space(1))) float*)(const_cast<char*>(a.plane[0]) + off) = s;
+            for (int l = 0; l < kQ; l++)
+                s += v[l];
+            if (s == a.never)
+                *(__attribute__((address_space(1))) E*)(const_cast<char*>(a.plane[0]) + off) =
+                    (E)s;
+        } else {
+            E s = 0;
+#pragma unroll
+            for (int l = 0; l < kQ; l++)
+                s += v[l];
+            if (s == (E)1e30f)   // populations sum to the density
+                *(__attribute__((address_space(1))) E*)(const_cast<char*>(a.plane[0]) + off) = s;
+        }
         return;
     }
 #pragma unroll
     for (int l = 0; l < kQ; l++) {
-        const float w = MODE == LBM_PROBE_WRITE ? weight<float>(l) + a.zero : v[l] + a.zero;
+        const E w = MODE == LBM_PROBE_WRITE ? keep(rest<E>(l), a) : keep(v[l], a);
         uint32_t o = off;
         asm volatile("" : "+v"(o));
-        __attribute__((address_space(1))) float* p =
-            (__attribute__((address_space(1))) float*)(const_cast<char*>(a.plane[l]) + o);
+        __attribute__((address_space(1))) E* p =
+            (__attribute__((address_space(1))) E*)(const_cast<char*>(a.plane[l]) + o);
         if (NT)
             __builtin_nontemporal_store(w, p);
         else
@@ -117,38 +149,99 @@ __global__ __launch_bounds__(kProbeBlock) void k_probe(ProbeArgs a)
     }
 }
 
+// BLOCK threads per workgroup, SPAN cells per thread one block apart: the
+// workgroup covers BLOCK * SPAN consecutive cells.
+template <class E, int MODE, bool NT, int BLOCK, int SPAN>
+__global__ __launch_bounds__(BLOCK) void k_probe(ProbeArgs a)
+{
+    const uint32_t base = a.first + blockIdx.x * (uint32_t)(BLOCK * SPAN) + threadIdx.x;
+#pragma unroll 1
+    for (int k = 0; k < SPAN; k++) {
+        const uint32_t id = base + k * BLOCK;
+        if (id >= a.last)
+            return;
+        probe_cell<E, MODE, NT>(a, id);
+    }
+}
+
+// Dynamic LDS that caps a CU at m resident workgroups (160 KB of LDS per CU).
+size_t cap_lds(int m)
+{
+    return m >= 4 ? 33 * 1024 : m == 3 ? 41 * 1024 : m == 2 ? 54 * 1024 : 81 * 1024;
+}
+
 template <int MODE, int VEC>
 hipError_t launch_vec(const ProbeArgs& a, int opts, size_t lds, hipStream_t st)
 {
     const uint32_t threads = (a.last - a.first) / VEC;
-    const uint32_t blocks = (threads + kProbeBlock - 1) / kProbeBlock;
+    const uint32_t blocks = (threads + 255) / 256;
     if (opts & LBM_LAUNCH_NONTEMPORAL)
-        hipLaunchKernelGGL((k_probe_vec<MODE, true, VEC>), dim3(blocks), dim3(kProbeBlock), lds,
-                           st, a);
+        hipLaunchKernelGGL((k_probe_vec<MODE, true, VEC>), dim3(blocks), dim3(256), lds, st, a);
     else
-        hipLaunchKernelGGL((k_probe_vec<MODE, false, VEC>), dim3(blocks), dim3(kProbeBlock), lds,
-                           st, a);
+        hipLaunchKernelGGL((k_probe_vec<MODE, false, VEC>), dim3(blocks), dim3(256), lds, st, a);
     return hipGetLastError();
 }
 
-template <int MODE>
+template <class E, int MODE, int BLOCK, int SPAN>
+hipError_t launch_span(const ProbeArgs& a, int opts, size_t lds, hipStream_t st)
+{
+    const uint32_t blocks = (a.last - a.first + BLOCK * SPAN - 1) / (BLOCK * SPAN);
+    if (opts & LBM_LAUNCH_NONTEMPORAL)
+        hipLaunchKernelGGL((k_probe<E, MODE, true, BLOCK, SPAN>), dim3(blocks), dim3(BLOCK), lds,
+                           st, a);
+    else
+        hipLaunchKernelGGL((k_probe<E, MODE, false, BLOCK, SPAN>), dim3(blocks), dim3(BLOCK),
+                           lds, st, a);
+    return hipGetLastError();
+}
+
+template <class E, int MODE, int BLOCK>
+hipError_t launch_block(const ProbeArgs& a, int opts, size_t lds, hipStream_t st)
+{
+    if (MODE == LBM_PROBE_READ || MODE == LBM_PROBE_WRITE)
+        return launch_span<E, MODE, BLOCK, 1>(a, opts, lds, st);
+    if (opts & LBM_PROBE_SPAN8)
+        return launch_span<E, MODE, BLOCK, 8>(a, opts, lds, st);
+    if (opts & LBM_PROBE_SPAN4)
+        return launch_span<E, MODE, BLOCK, 4>(a, opts, lds, st);
+    if (opts & LBM_PROBE_SPAN2)
+        return launch_span<E, MODE, BLOCK, 2>(a, opts, lds, st);
+    return launch_span<E, MODE, BLOCK, 1>(a, opts, lds, st);
+}
+
+template <class E, int MODE>
 hipError_t launch_mode(const ProbeArgs& a, int opts, hipStream_t st)
 {
-    const size_t lds = (opts & LBM_LAUNCH_WAVES2)   ? 54 * 1024
-                       : (opts & LBM_LAUNCH_WAVES3) ? 41 * 1024
-                       : (opts & LBM_LAUNCH_WAVES4) ? 33 * 1024 : 0;
-    if (MODE != LBM_PROBE_READ && MODE != LBM_PROBE_WRITE) {
-        if (opts & LBM_PROBE_VEC4)
-            return launch_vec<MODE, 4>(a, opts, lds, st);
-        if (opts & LBM_PROBE_VEC2)
-            return launch_vec<MODE, 2>(a, opts, lds, st);
+    const int block = (opts & LBM_PROBE_BLOCK1024) ? 1024 : (opts & LBM_PROBE_BLOCK512) ? 512 : 256;
+    const int waves = (opts & LBM_LAUNCH_WAVES2) ? 2 : (opts & LBM_LAUNCH_WAVES3) ? 3
+                      : (opts & LBM_LAUNCH_WAVES4) ? 4 : 0;
+    // a cap of w waves per SIMD leaves w * 256 / block workgroups per CU
+    const size_t lds = waves ? cap_lds(waves * 256 / block) : 0;
+    if constexpr (sizeof(E) == 4) {
+        if (MODE != LBM_PROBE_READ && MODE != LBM_PROBE_WRITE) {
+            if (opts & LBM_PROBE_VEC4)
+                return launch_vec<MODE, 4>(a, opts, lds, st);
+            if (opts & LBM_PROBE_VEC2)
+                return launch_vec<MODE, 2>(a, opts, lds, st);
+        }
     }
-    const uint32_t blocks = (a.last - a.first + kProbeBlock - 1) / kProbeBlock;
-    if (opts & LBM_LAUNCH_NONTEMPORAL)
-        hipLaunchKernelGGL((k_probe<MODE, true>), dim3(blocks), dim3(kProbeBlock), lds, st, a);
-    else
-        hipLaunchKernelGGL((k_probe<MODE, false>), dim3(blocks), dim3(kProbeBlock), lds, st, a);
-    return hipGetLastError();
+    if (block == 1024)
+        return launch_block<E, MODE, 1024>(a, opts, lds, st);
+    if (block == 512)
+        return launch_block<E, MODE, 512>(a, opts, lds, st);
+    return launch_block<E, MODE, 256>(a, opts, lds, st);
+}
+
+template <class E>
+hipError_t launch_probe(const ProbeArgs& a, int mode, int opts, hipStream_t st)
+{
+    switch (mode) {
+    case LBM_PROBE_RMW_ALIGNED: return launch_mode<E, LBM_PROBE_RMW_ALIGNED>(a, opts, st);
+    case LBM_PROBE_READ: return launch_mode<E, LBM_PROBE_READ>(a, opts, st);
+    case LBM_PROBE_WRITE: return launch_mode<E, LBM_PROBE_WRITE>(a, opts, st);
+    default:   // the shifted variants differ only in their plane bases
+        return launch_mode<E, LBM_PROBE_RMW_SHIFTED>(a, opts, st);
+    }
 }
 
 }  // namespace
@@ -160,17 +253,43 @@ using namespace lbm;
 extern "C" int lbm_probe_pattern(lbm_solver* s, int mode, int options, uint64_t* cells,
                                  hipStream_t stream)
 {
+    const int known = LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_WAVES4 | LBM_LAUNCH_WAVES3 |
+                      LBM_LAUNCH_WAVES2 | LBM_PROBE_VEC2 | LBM_PROBE_VEC4 | LBM_PROBE_BLOCK512 |
+                      LBM_PROBE_BLOCK1024 | LBM_PROBE_SPAN2 | LBM_PROBE_SPAN4 | LBM_PROBE_SPAN8;
     if (!s || mode < LBM_PROBE_RMW_ALIGNED || mode > LBM_PROBE_RMW_SHIFTED_YZ ||
-        (options & ~(LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_WAVES4 | LBM_LAUNCH_WAVES3 |
-                     LBM_LAUNCH_WAVES2 | LBM_PROBE_VEC2 | LBM_PROBE_VEC4))) {
+        (options & ~known)) {
         set_error("lbm_probe_pattern: invalid argument");
         return LBM_ERR_ARG;
     }
-    if (s->p.precision != LBM_F32) {
-        set_error("lbm_probe_pattern: FP32 solvers only");
+    const bool rmw = mode != LBM_PROBE_READ && mode != LBM_PROBE_WRITE;
+    const int vec = options & (LBM_PROBE_VEC2 | LBM_PROBE_VEC4);
+    const int shape = options & (LBM_PROBE_BLOCK512 | LBM_PROBE_BLOCK1024 | LBM_PROBE_SPAN2 |
+                                 LBM_PROBE_SPAN4 | LBM_PROBE_SPAN8);
+    const auto one_bit = [](int v) { return (v & (v - 1)) == 0; };
+    const int spans = options & (LBM_PROBE_SPAN2 | LBM_PROBE_SPAN4 | LBM_PROBE_SPAN8);
+    const int blocks = options & (LBM_PROBE_BLOCK512 | LBM_PROBE_BLOCK1024);
+    if (!one_bit(vec) || !one_bit(spans) || !one_bit(blocks) || (vec && shape) ||
+        (!rmw && (vec || shape))) {
+        set_error("lbm_probe_pattern: VEC2 / VEC4, BLOCK512 / BLOCK1024 and SPAN2 / 4 / 8 "
+                  "each pick one, vector accesses exclude the workgroup shapes, and the "
+                  "read / write streams take neither");
+        return LBM_ERR_ARG;
+    }
+    const int block = (options & LBM_PROBE_BLOCK1024) ? 1024 : (options & LBM_PROBE_BLOCK512) ? 512
+                                                                                              : 256;
+    const int waves = (options & LBM_LAUNCH_WAVES2) ? 2 : (options & LBM_LAUNCH_WAVES3) ? 3
+                      : (options & LBM_LAUNCH_WAVES4) ? 4 : 0;
+    if (waves && waves * 256 < block) {
+        set_error("lbm_probe_pattern: that waves-per-SIMD cap holds less than one "
+                  "workgroup of this size");
+        return LBM_ERR_ARG;
+    }
+    const size_t elem = (size_t)s->elem_bytes;
+    if (vec && s->p.precision != LBM_F32) {
+        set_error("lbm_probe_pattern: vector accesses on FP32 solvers only");
         return LBM_ERR_UNSUPPORTED;
     }
-    if ((uint64_t)s->stride * sizeof(float) >= (1ull << 32)) {
+    if ((uint64_t)s->stride * elem >= (1ull << 32)) {
         set_error("lbm_probe_pattern: planes beyond 32-bit offsets");
         return LBM_ERR_UNSUPPORTED;
     }
@@ -199,23 +318,18 @@ extern "C" int lbm_probe_pattern(lbm_solver* s, int mode, int options, uint64_t*
     a.first = (uint32_t)span;
     a.last = (uint32_t)(s->n - span);
     a.zero = 0.0f;
+    a.zero_bits = 0;
+    a.never = 0xFFFFFFFFu;
     for (int l = 0; l < kQ; l++)
-        a.plane[l] = (const char*)s->dd +
-                     ((int64_t)l * s->stride + shift[l]) * (int64_t)sizeof(float);
-    const bool vec = mode != LBM_PROBE_READ && mode != LBM_PROBE_WRITE;
-    const uint32_t per = vec ? ((options & LBM_PROBE_VEC4) ? 4 : (options & LBM_PROBE_VEC2) ? 2 : 1)
-                             : 1;
+        a.plane[l] = (const char*)s->dd + ((int64_t)l * s->stride + shift[l]) * (int64_t)elem;
+    const uint32_t per = (options & LBM_PROBE_VEC4) ? 4 : (options & LBM_PROBE_VEC2) ? 2 : 1;
     if (cells)
         *cells = (a.last - a.first) / per * per;
-    hipError_t e = hipErrorInvalidValue;
-    switch (mode) {
-    case LBM_PROBE_RMW_ALIGNED: e = launch_mode<LBM_PROBE_RMW_ALIGNED>(a, options, stream); break;
-    case LBM_PROBE_RMW_SHIFTED: e = launch_mode<LBM_PROBE_RMW_SHIFTED>(a, options, stream); break;
-    case LBM_PROBE_READ: e = launch_mode<LBM_PROBE_READ>(a, options, stream); break;
-    case LBM_PROBE_WRITE: e = launch_mode<LBM_PROBE_WRITE>(a, options, stream); break;
-    default:   // the shifted variants differ only in their plane bases
-        e = launch_mode<LBM_PROBE_RMW_SHIFTED>(a, options, stream);
-        break;
+    hipError_t e;
+    switch (elem) {
+    case 8: e = launch_probe<double>(a, mode, options, stream); break;
+    case 2: e = launch_probe<uint16_t>(a, mode, options, stream); break;
+    default: e = launch_probe<float>(a, mode, options, stream); break;
     }
     if (e != hipSuccess)
         return hip_fail(e, "probe kernel launch");

@@ -266,16 +266,29 @@ int64_t lbm_face_elements(const lbm_solver* solver, int direction);
 
 /* Pack one face into a dense device buffer [l][z][y][x], the device-side
  * half of the exchange in CController::stepAlpha / stepBeta
- * (CController.cpp:369-570, 807-1008):
- *   phase 0 (after alpha): the send plane (offset 1), all 19 populations;
- *   phase 1 (after beta) : the halo plane, all 19 populations. */
+ * (CController.cpp:369-570, 807-1008). Each phase moves the 5 planes the
+ * receiver's interior depends on; the buffer keeps the 19-plane layout and
+ * the other 14 planes are not written:
+ *   phase 0 (after alpha): the send plane (offset 1), the planes whose lattice
+ *            vectors point back into this solver through the face (the
+ *            reference sends all 19, CController.cpp:380-387);
+ *   phase 1 (after beta) : the halo plane, the planes whose vectors point out
+ *            through the face, i.e. the populations the neighbour's
+ *            direction-filtered unpack applies (the reference sends all 19
+ *            and filters on receipt, :818-825, 849-874).
+ * In the beta step plane l at cell p is read and written only by cell
+ * p - c_l, so the other 14 planes of a halo plane never reach an interior
+ * cell before the phase-1 unpack overwrites where they went: every interior
+ * cell is bit-identical to the reference's 19-plane exchange. */
 int lbm_pack_face(lbm_solver* solver, int direction, int phase, void* dev_buf,
                   hipStream_t stream);
 
-/* Unpack a buffer packed by the neighbour across face `direction`:
- *   phase 0: into the halo plane, all 19 populations (CController.cpp:411-435);
- *   phase 1: into the send plane, only the populations pointing into the
- *            domain (setDensityDistributions with direction, :849-874). */
+/* Unpack a buffer packed by the neighbour across face `direction` (the
+ * receiving face; the sender packed the opposite one), the same 5 planes:
+ *   phase 0: into the halo plane, the planes whose vectors point out through
+ *            the face (CController.cpp:411-435 writes all 19);
+ *   phase 1: into the send plane, the populations pointing into the domain
+ *            (setDensityDistributions with direction, :849-874). */
 int lbm_unpack_face(lbm_solver* solver, int direction, int phase, const void* dev_buf,
                     hipStream_t stream);
 
@@ -292,8 +305,8 @@ int lbm_unpack_face_pair(lbm_solver* solver, int axis, int phase, const void* lo
 /* Fused halo packing of the two x faces into caller-owned device buffers of
  * lbm_face_elements(LEFT) elements each (NULL disables a side, both NULL
  * turns it off). While set, every whole-domain or box step also writes:
- *   alpha: all 19 populations of the send plane (x = 1 / x = nx_h - 2) into
- *          the buffer, exactly what lbm_pack_face(phase 0) produces;
+ *   alpha: the 5 populations of the send plane (x = 1 / x = nx_h - 2) that
+ *          lbm_pack_face(phase 0) packs, at the same buffer positions;
  *   beta : the 5 populations that cross the face into the halo plane, at
  *          their halo-plane positions, i.e. the entries of
  *          lbm_pack_face(phase 1) that the neighbour's
@@ -443,8 +456,9 @@ int lbm_device_bytes(const lbm_params* params, int pair_buffer, uint64_t* bytes)
 /* ---- measurement --------------------------------------------------------
  * No reference counterpart: bandwidth probes of the step kernels' access
  * patterns without the collision, the ceilings that bound k_alpha / k_beta on
- * the device (bench.py --workload ceiling). They run on an FP32 solver's own
- * population array, one thread per cell: */
+ * the device (bench.py --workload ceiling). They run on a solver's own
+ * population array, elements of its storage size (4 bytes FP32, 8 FP64, 2 for
+ * the 16-bit formats), one thread per cell: */
 enum lbm_probe_mode {
     LBM_PROBE_RMW_ALIGNED = 0,  /* load + store the cell's own 19 slots (alpha's pattern) */
     LBM_PROBE_RMW_SHIFTED = 1,  /* load + store plane l at id + c_l (beta's pattern), cells
@@ -457,12 +471,22 @@ enum lbm_probe_mode {
     LBM_PROBE_RMW_SHIFTED_X = 5,  /* the x components of beta's shifts only */
     LBM_PROBE_RMW_SHIFTED_YZ = 6  /* the y and z components only */
 };
-/* options: LBM_LAUNCH_NONTEMPORAL and LBM_LAUNCH_WAVES4/3/2, and for the
- * read-modify-write patterns LBM_PROBE_VEC2 / LBM_PROBE_VEC4 (2 or 4
- * consecutive cells per thread, one vector access per plane); *cells (may be
- * NULL) receives the number of cells the launch covers. Asynchronous. */
+/* options: LBM_LAUNCH_NONTEMPORAL and LBM_LAUNCH_WAVES4/3/2 (a cap must hold
+ * one workgroup: WAVES3 / WAVES2 exclude BLOCK1024), and for the
+ * read-modify-write patterns either the workgroup shape -- LBM_PROBE_BLOCK512
+ * / BLOCK1024 threads (default 256) and LBM_PROBE_SPAN2 / 4 / 8 cells per
+ * thread one block apart (default 1), so a workgroup covers block x span
+ * consecutive cells -- or, on FP32 solvers, LBM_PROBE_VEC2 / LBM_PROBE_VEC4
+ * (2 or 4 consecutive cells per thread, one vector access per plane); *cells
+ * (may be NULL) receives the number of cells the launch covers.
+ * Asynchronous. */
 #define LBM_PROBE_VEC2 256
 #define LBM_PROBE_VEC4 512
+#define LBM_PROBE_BLOCK512 1024
+#define LBM_PROBE_BLOCK1024 2048
+#define LBM_PROBE_SPAN2 4096
+#define LBM_PROBE_SPAN4 8192
+#define LBM_PROBE_SPAN8 16384
 int lbm_probe_pattern(lbm_solver* solver, int mode, int options, uint64_t* cells,
                       hipStream_t stream);
 

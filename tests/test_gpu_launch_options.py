@@ -134,42 +134,45 @@ def test_box_steps_same_bits_under_wide_workgroups(hip, precision):
             assert x.tobytes() == y.tobytes(), (opts, i)
 
 
-def test_pattern_probes_keep_or_reset_the_state(hip):
+@pytest.mark.parametrize("prec", ["f32", "f16s", "f64"])
+def test_pattern_probes_keep_or_reset_the_state(hip, prec):
     """lbm_probe_pattern (the bandwidth probes of bench.py --workload
-    ceiling): the read-modify-write and read patterns leave every population
-    as it was, the write pattern stores the rest state; bad arguments and
-    non-FP32 solvers are rejected."""
+    ceiling), on the solver's own elements (4, 2 or 8 bytes): the
+    read-modify-write and read patterns leave every population as it was
+    under every workgroup shape, the write pattern stores the rest state;
+    bad arguments and option combinations are rejected."""
     import ctypes
     from cavityflow_cuda_amd import LbmError
     from cavityflow_cuda_amd._lib import check, lib
-    print("[case] pattern probes", flush=True)
-    s, full = state(hip, "f32", np.random.default_rng(3))
+    print(f"[case] pattern probes {prec}", flush=True)
+    s, full = state(hip, prec, np.random.default_rng(3))
     before = s.getDensityDistributions(*full).copy()
     nx, ny, _ = s.size_h
     pitch = (nx + 31) // 32 * 32
     span = {0: 0, 1: nx * ny + nx, 2: 0, 4: pitch * ny + pitch, 5: 1, 6: nx * ny + nx}
+    vec = (256, 512, 512 | 17, 256 | 4) if prec == "f32" else ()
+    shapes = (1024, 2048 | 4, 4096, 8192 | 1024 | 1, 16384 | 2048, 16384 | 17, 4096 | 1024 | 16)
     for mode in (0, 1, 2, 4, 5, 6):
-        for opt in (0, 1, 16, 17, 8, 4, 256, 512, 512 | 17, 256 | 4):
+        opts = (0, 1, 16, 17, 8, 4) + (() if mode == 2 else vec + shapes)
+        for opt in opts:
             n = ctypes.c_uint64()
             check(lib().lbm_probe_pattern(s.handle, mode, opt, ctypes.byref(n), None))
             s.sync()
-            per = 1 if mode == 2 else (4 if opt & 512 else (2 if opt & 256 else 1))
+            per = 4 if opt & 512 else (2 if opt & 256 else 1)
             cells = int(np.prod(s.size_h)) - 2 * span[mode]
             assert n.value == cells // per * per, (mode, opt)
             assert s.getDensityDistributions(*full).tobytes() == before.tobytes(), (mode, opt)
     check(lib().lbm_probe_pattern(s.handle, 3, 1, None, None))
     s.sync()
-    w = np.array([1 / 18] * 4 + [1 / 36] * 12 + [1 / 18] * 2 + [1 / 3], np.float32)
+    dt = np.float64 if prec == "f64" else np.float32
+    w = np.array([1 / 18] * 4 + [1 / 36] * 12 + [1 / 18] * 2 + [1 / 3], dt)
     got = s.getDensityDistributions(*full).reshape(19, -1)
     assert np.array_equal(got, np.repeat(w[:, None], got.shape[1], axis=1))
-    with pytest.raises(LbmError):
-        check(lib().lbm_probe_pattern(s.handle, 7, 0, None, None))
-    with pytest.raises(LbmError):
-        check(lib().lbm_probe_pattern(s.handle, 0, 2, None, None))
-    with pytest.raises(LbmError):
-        check(lib().lbm_probe_pattern(s.handle, 0, 1024, None, None))
+    bad = [(7, 0), (0, 2), (0, 32768), (0, 256 | 4096), (0, 1024 | 2048), (0, 4096 | 8192),
+           (2, 4096), (3, 1024), (0, 2048 | 16), (0, 2048 | 8)]
+    if prec != "f32":
+        bad.append((0, 256))
+    for mode, opt in bad:
+        with pytest.raises(LbmError):
+            check(lib().lbm_probe_pattern(s.handle, mode, opt, None, None))
     s.close()
-    d = hip.LbmSolverHIP((8, 8, 8), precision="f64")
-    with pytest.raises(LbmError):
-        check(lib().lbm_probe_pattern(d.handle, 0, 0, None, None))
-    d.close()

@@ -245,6 +245,11 @@ def test_box_errors(hip):
 
 
 def test_pack_unpack_faces_match_oracle(hip, oracle_cpu):
+    """lbm_pack_face writes exactly the 5 planes of its face and phase
+    (phase 0: the vectors pointing back into the sender; phase 1: those
+    pointing out of it) as the oracle's box copy reads them, and leaves the
+    other 14 planes of the buffer alone; lbm_unpack_face writes those planes
+    of the receiving face as the oracle's direction-filtered box copy does."""
     from hiprt import DeviceBuffer
     say("pack/unpack faces")
     rng = np.random.default_rng(12)
@@ -256,19 +261,27 @@ def test_pack_unpack_faces_match_oracle(hip, oracle_cpu):
     for direction in range(6):
         n = s.face_elements(direction)
         for phase in (0, 1):
+            send = oracle_cpu.direction_mask(direction if phase == 0 else direction ^ 1)
             buf = DeviceBuffer(4 * n)
+            buf.upload(np.full(n, -7.0, np.float32))
             s.pack_face(direction, phase, buf)
             s.sync()
             origin, box = s.face_box(direction, 0 if phase == 0 else 1)
-            packed = buf.download(np.float32, n)
-            assert packed.tobytes() == o.getDensityDistributions(origin, box).tobytes()
+            packed = buf.download(np.float32, n).reshape(19, -1)
+            want = o.getDensityDistributions(origin, box).reshape(19, -1)
+            for l in range(19):
+                if (send >> l) & 1:
+                    assert packed[l].tobytes() == want[l].tobytes(), (direction, phase, l)
+                else:
+                    assert (packed[l] == -7.0).all(), (direction, phase, l)
             src = rng.random(n).astype(np.float32)
             buf.upload(src)
+            # the receiving face `direction` takes what its neighbour's opposite face sent
             s.unpack_face(direction, phase, buf)
             s.sync()
             buf.free()
             origin, box = s.face_box(direction, 1 if phase == 0 else 0)
-            o.setDensityDistributions(origin, box, src, None if phase == 0 else direction)
+            o.setDensityDistributions(origin, box, src, direction ^ 1 if phase == 0 else direction)
             assert s.getDensityDistributions((0, 0, 0), s.size_h).tobytes() == o.dd.tobytes()
     s.close()
 
