@@ -19,6 +19,7 @@ LAUNCH_NONTEMPORAL, LAUNCH_XCD_REMAP, LAUNCH_WAVES4, LAUNCH_WAVES3, LAUNCH_WAVES
 LAUNCH_ADDR64 = 32
 LAUNCH_WIDE = 64
 LAUNCH_LDS = 128
+LAUNCH_BIG = 256
 LBM_OK = 0
 NOALLOC_MAX_BOXES = 16  # LBM_NOALLOC_MAX_BOXES
 (LBM_ERR_ARG, LBM_ERR_BOX, LBM_ERR_HIP, LBM_ERR_NOMEM, LBM_ERR_PARAMS, LBM_ERR_STATE,

@@ -458,7 +458,7 @@ int lbm_set_launch_options(lbm_solver* s, int kernel, int options)
     if (!s || kernel < 0 || kernel > 1 ||
         (options & ~(LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_XCD_REMAP | LBM_LAUNCH_WAVES4 |
                      LBM_LAUNCH_WAVES3 | LBM_LAUNCH_WAVES2 | LBM_LAUNCH_ADDR64 |
-                     LBM_LAUNCH_WIDE | LBM_LAUNCH_LDS))) {
+                     LBM_LAUNCH_WIDE | LBM_LAUNCH_LDS | LBM_LAUNCH_BIG))) {
         set_error("lbm_set_launch_options: invalid argument");
         return LBM_ERR_ARG;
     }

@@ -239,6 +239,17 @@ constexpr int kBatchOptions =
     LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_XCD_REMAP | LBM_LAUNCH_WAVES4 | LBM_LAUNCH_WAVES3 |
     LBM_LAUNCH_WAVES2;
 
+// Launch options a batch starts from before any autotuning (kernel 0 =
+// alpha, 1 = beta): the single-domain options of 256-thread workgroups
+// measured with the exchange for FP32 / F64, nontemporal alpha and plain
+// beta for the 16-bit formats.
+static int batch_start_options(int precision, int kernel)
+{
+    if (precision == LBM_F32 || precision == LBM_F64)
+        return default_launch(precision, kernel, true, false) & kBatchOptions;
+    return kernel == 0 ? LBM_LAUNCH_NONTEMPORAL : 0;
+}
+
 // Unused LDS reserved per workgroup for LBM_LAUNCH_WAVES4 / WAVES3 / WAVES2:
 // caps the resident workgroups of a CU's 160 KB at 4 / 3 / 2 waves per SIMD.
 size_t waves_lds(int opts, int block)
@@ -586,11 +597,10 @@ int lbm_batch_create(lbm_solver* const* members, int count, lbm_batch** out)
     t.block = 256;
     t.cells_per_thread = 1;
     t.waves_per_cu = 0;
-    // the single-domain defaults of 256-thread workgroups measured with the
-    // exchange; the batch kernels take their shape from the tuning (block,
+    // the batch kernels take their shape from the tuning (block,
     // cells_per_thread)
-    t.options[0] = default_launch(b->precision, 0, true, false) & kBatchOptions;
-    t.options[1] = default_launch(b->precision, 1, true, false) & kBatchOptions;
+    t.options[0] = batch_start_options(b->precision, 0);
+    t.options[1] = batch_start_options(b->precision, 1);
     t.pairs = 0;
     t.group_mb = 0;
     if ((rc = apply_tuning(b, t))) {

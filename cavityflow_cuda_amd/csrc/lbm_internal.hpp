@@ -158,7 +158,9 @@ hipError_t launch_reduce_boxes(const lbm_solver* const* s, const Box* b, int cou
 // about 160^3 they leave too few workgroups), for alpha only when it does
 // not pack the x faces (with_pack: the fused pack stores cost those
 // workgroups more than they gain; FP32 alpha then keeps 3 waves per SIMD,
-// the best with the six-face exchange at 512^3).
+// the best with the six-face exchange at 512^3). 16-bit storage: cached
+// accesses; F16S and BF16S beta in 1024-thread workgroups on large domains
+// (F16C beta, with more integer work per cell, loses with them).
 constexpr int64_t kWideCells = 6 << 20;   // halo-inclusive cells for `wide`
 
 inline int default_launch(int precision, int kernel, bool with_pack, bool wide)
@@ -174,8 +176,10 @@ inline int default_launch(int precision, int kernel, bool with_pack, bool wide)
     case LBM_F64:
         return kernel == 0 ? (LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_WAVES2)
                            : (LBM_LAUNCH_LDS | LBM_LAUNCH_WAVES2);
+    case LBM_F16C:
+        return 0;
     default:
-        return kernel == 0 ? LBM_LAUNCH_NONTEMPORAL : 0;
+        return kernel == 1 && wide ? LBM_LAUNCH_BIG : 0;
     }
 }
 

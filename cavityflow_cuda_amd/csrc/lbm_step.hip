@@ -106,21 +106,25 @@ __global__ __launch_bounds__(kBlock) void k_init(StepArgs<typename P::C> a, int 
         a.rho[id] = rho;
 }
 
-// WIDE launches run 512-thread workgroups over 1024 consecutive cells, two
-// cells per thread one block apart; otherwise one cell per thread of a
-// 256-thread workgroup.
-template <bool WIDE>
+// Workgroup shapes: kShapeBase one cell per thread of a 256-thread workgroup;
+// kShapeWide (LBM_LAUNCH_WIDE) 512-thread workgroups over 1024 consecutive
+// cells, two cells per thread one block apart; kShapeBig (LBM_LAUNCH_BIG)
+// 1024-thread workgroups, one cell per thread.
+constexpr int kShapeBase = 0, kShapeWide = 1, kShapeBig = 2;
+template <int SHAPE>
 struct Shape {
-    static constexpr uint32_t kThreads = WIDE ? 2 * kBlock : kBlock;
-    static constexpr uint32_t kCells = WIDE ? 2 : 1;          // per thread
-    static constexpr uint32_t kSpan = kThreads * kCells;      // per workgroup
+    static constexpr uint32_t kThreads = SHAPE == kShapeWide  ? 2 * kBlock
+                                         : SHAPE == kShapeBig ? 4 * kBlock
+                                                              : kBlock;
+    static constexpr uint32_t kCells = SHAPE == kShapeWide ? 2 : 1;   // per thread
+    static constexpr uint32_t kSpan = kThreads * kCells;              // per workgroup
 };
 
-template <class P, bool FULL, bool OFF32, bool NT, bool WIDE>
-__global__ __launch_bounds__(Shape<WIDE>::kThreads) void k_alpha(StepArgs<typename P::C> a)
+template <class P, bool FULL, bool OFF32, bool NT, int SHAPE>
+__global__ __launch_bounds__(Shape<SHAPE>::kThreads) void k_alpha(StepArgs<typename P::C> a)
 {
     using S = typename P::S;
-    using G = Shape<WIDE>;
+    using G = Shape<SHAPE>;
     const uint32_t blk = a.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     for (uint32_t k = 0; k < G::kCells; k++) {
         const uint32_t gid = blk * G::kSpan + k * G::kThreads + threadIdx.x;
@@ -137,11 +141,11 @@ __global__ __launch_bounds__(Shape<WIDE>::kThreads) void k_alpha(StepArgs<typena
     }
 }
 
-template <class P, bool FULL, bool OFF32, bool NT, bool WIDE>
-__global__ __launch_bounds__(Shape<WIDE>::kThreads) void k_beta(StepArgs<typename P::C> a)
+template <class P, bool FULL, bool OFF32, bool NT, int SHAPE>
+__global__ __launch_bounds__(Shape<SHAPE>::kThreads) void k_beta(StepArgs<typename P::C> a)
 {
     using S = typename P::S;
-    using G = Shape<WIDE>;
+    using G = Shape<SHAPE>;
     const uint32_t blk = a.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     // One decision per workgroup: neighbours of every cell of this block stay
     // inside [0, n) unless the block touches the first or last slab (box_cell
@@ -325,48 +329,54 @@ constexpr size_t kWaves3Lds = 41 * 1024;
 constexpr size_t kWaves2Lds = 54 * 1024;
 
 // Reserved LDS for a waves-per-SIMD cap with WIDE (512-thread) workgroups:
-// 4 waves per SIMD leave room for 2 workgroups per CU, 3 or 2 for one.
+// 4 waves per SIMD leave room for 2 workgroups per CU, 3 or 2 for one. A BIG
+// (1024-thread) workgroup alone fills 4 waves per SIMD: every cap holds one.
 constexpr size_t kWideWaves4Lds = 54 * 1024;
 constexpr size_t kWideWaves2Lds = 81 * 1024;
 
-template <class P, template <class, bool, bool, bool, bool> class K, bool FULL, bool WIDE>
+template <class P, template <class, bool, bool, bool, int> class K, bool FULL, int SHAPE>
 static void launch_shape(const lbm_solver* s, const StepArgs<typename P::C>& a, int opts,
                          hipStream_t st)
 {
-    using G = Shape<WIDE>;
+    using G = Shape<SHAPE>;
     const uint32_t blocks = (uint32_t)(((uint64_t)a.cells + G::kSpan - 1) / G::kSpan);
     const bool cap2 = opts & (LBM_LAUNCH_WAVES2 | LBM_LAUNCH_WAVES3);
-    const size_t lds = WIDE ? (cap2 ? kWideWaves2Lds
-                               : (opts & LBM_LAUNCH_WAVES4) ? kWideWaves4Lds : 0)
-                            : (opts & LBM_LAUNCH_WAVES2)   ? kWaves2Lds
-                            : (opts & LBM_LAUNCH_WAVES3) ? kWaves3Lds
-                            : (opts & LBM_LAUNCH_WAVES4) ? kWaves4Lds : 0;
+    const bool any_cap = opts & (LBM_LAUNCH_WAVES2 | LBM_LAUNCH_WAVES3 | LBM_LAUNCH_WAVES4);
+    const size_t lds = SHAPE == kShapeBig  ? (any_cap ? kWideWaves2Lds : 0)
+                       : SHAPE == kShapeWide ? (cap2                           ? kWideWaves2Lds
+                                                : (opts & LBM_LAUNCH_WAVES4) ? kWideWaves4Lds
+                                                                             : 0)
+                       : (opts & LBM_LAUNCH_WAVES2) ? kWaves2Lds
+                       : (opts & LBM_LAUNCH_WAVES3) ? kWaves3Lds
+                       : (opts & LBM_LAUNCH_WAVES4) ? kWaves4Lds : 0;
     const dim3 grid(blocks), block(G::kThreads);
     if (!off32(s) || (opts & LBM_LAUNCH_ADDR64))
-        hipLaunchKernelGGL((K<P, FULL, false, false, WIDE>::kernel), grid, block, lds, st, a);
+        hipLaunchKernelGGL((K<P, FULL, false, false, SHAPE>::kernel), grid, block, lds, st, a);
     else if (opts & LBM_LAUNCH_NONTEMPORAL)
-        hipLaunchKernelGGL((K<P, FULL, true, true, WIDE>::kernel), grid, block, lds, st, a);
+        hipLaunchKernelGGL((K<P, FULL, true, true, SHAPE>::kernel), grid, block, lds, st, a);
     else
-        hipLaunchKernelGGL((K<P, FULL, true, false, WIDE>::kernel), grid, block, lds, st, a);
+        hipLaunchKernelGGL((K<P, FULL, true, false, SHAPE>::kernel), grid, block, lds, st, a);
 }
 
-template <class P, template <class, bool, bool, bool, bool> class K, bool FULL>
+template <class P, template <class, bool, bool, bool, int> class K, bool FULL>
 static void launch_variant(const lbm_solver* s, const StepArgs<typename P::C>& a, int opts,
                            hipStream_t st)
 {
-    if (opts & LBM_LAUNCH_WIDE)
-        launch_shape<P, K, FULL, true>(s, a, opts, st);
+    if (opts & LBM_LAUNCH_BIG)
+        launch_shape<P, K, FULL, kShapeBig>(s, a, opts, st);
+    else if (opts & LBM_LAUNCH_WIDE)
+        launch_shape<P, K, FULL, kShapeWide>(s, a, opts, st);
     else
-        launch_shape<P, K, FULL, false>(s, a, opts, st);
+        launch_shape<P, K, FULL, kShapeBase>(s, a, opts, st);
 }
 
-template <class P, bool FULL, bool OFF32, bool NT, bool WIDE>
+template <class P, bool FULL, bool OFF32, bool NT, int SHAPE>
 struct AlphaK {
-    static constexpr auto kernel = k_alpha<P, FULL, OFF32, NT, WIDE>;
+    static constexpr auto kernel = k_alpha<P, FULL, OFF32, NT, SHAPE>;
 };
-template <class P, bool FULL, bool OFF32, bool NT, bool WIDE>
+template <class P, bool FULL, bool OFF32, bool NT, int SHAPE>
 struct BetaK {
-    static constexpr auto kernel = k_beta<P, FULL, OFF32, NT, WIDE>;
+    static constexpr auto kernel = k_beta<P, FULL, OFF32, NT, SHAPE>;
 };
 
 template <class P>

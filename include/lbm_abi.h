@@ -178,7 +178,9 @@ int lbm_simulation_step_beta(lbm_solver* solver, const int origin[3], const int 
  * interior cells) and WAVES2 below; F32 alpha NONTEMPORAL | WAVES3 with a step
  * pack (lbm_set_step_pack), else NONTEMPORAL | WAVES2, plus WIDE on those
  * large domains; F64 alpha NONTEMPORAL | WAVES2, beta LDS | WAVES2; F16S /
- * BF16S alpha NONTEMPORAL, beta none. No reference
+ * BF16S alpha none, beta BIG on those large domains (else none); F16C none.
+ * Batches start from F32 / F64 alpha and beta as for 256-thread workgroups
+ * with a step pack, 16-bit alpha NONTEMPORAL and beta none. No reference
  * counterpart: the reference takes its block shapes from the XML
  * configuration (default.xml:93-115). */
 enum lbm_launch_option {
@@ -195,10 +197,15 @@ enum lbm_launch_option {
                                     (1024 consecutive cells per workgroup; with it
                                     WAVES3 caps at 2 waves per SIMD, as WAVES2;
                                     single-domain kernels only) */
-    LBM_LAUNCH_LDS = 128         /* beta of the whole domain stages its shifted
+    LBM_LAUNCH_LDS = 128,        /* beta of the whole domain stages its shifted
                                     plane ranges through LDS with line-aligned
-                                    accesses (wins over WIDE; alpha and box steps
-                                    ignore it) */
+                                    accesses (wins over WIDE and BIG; alpha and box
+                                    steps ignore it) */
+    LBM_LAUNCH_BIG = 256         /* 1024-thread workgroups, one cell per thread
+                                    (wins over WIDE; a workgroup alone fills 4
+                                    waves per SIMD, so any WAVES cap holds one per
+                                    CU; single-domain kernels only; F64 beta spills
+                                    registers in this shape) */
 };
 int lbm_set_launch_options(lbm_solver* solver, int kernel, int options);
 int lbm_get_launch_options(const lbm_solver* solver, int kernel, int* options);

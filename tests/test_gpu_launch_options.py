@@ -45,9 +45,9 @@ def run(s, full, alpha_opt, beta_opt):
 def test_every_launch_option_gives_the_same_bits(hip, precision):
     print("[case] launch options", precision, flush=True)
     want = None
-    for opt in range(256):
+    for opt in range(512):
         s, full = state(hip, precision, np.random.default_rng(11))
-        got = run(s, full, opt, (opt * 5) % 256)
+        got = run(s, full, opt, (opt * 5) % 512)
         s.close()
         if want is None:
             want = got
@@ -57,8 +57,8 @@ def test_every_launch_option_gives_the_same_bits(hip, precision):
 
 
 # per-precision defaults (alpha, beta) documented in include/lbm_abi.h
-DEFAULTS = {"f32": (1 | 16, 16), "f64": (1 | 16, 128 | 16), "f16s": (1, 0), "bf16s": (1, 0),
-            "f16c": (1, 0)}
+DEFAULTS = {"f32": (1 | 16, 16), "f64": (1 | 16, 128 | 16), "f16s": (0, 0), "bf16s": (0, 0),
+            "f16c": (0, 0)}
 
 
 @pytest.mark.parametrize("precision", sorted(DEFAULTS))
@@ -72,7 +72,7 @@ def test_default_options_are_valid_and_settable(hip, precision):
         check(lib().lbm_get_launch_options(s.handle, k, ctypes.byref(v)))
         assert v.value == DEFAULTS[precision][k]
     with pytest.raises(LbmError):
-        check(lib().lbm_set_launch_options(s.handle, 0, 256))
+        check(lib().lbm_set_launch_options(s.handle, 0, 512))
     # with a step pack FP32 alpha keeps 3 waves per SIMD; options the caller
     # set are kept; without the pack the first defaults return
     want_pack = {"f32": (1 | 8, 16)}.get(precision, DEFAULTS[precision])
@@ -91,14 +91,18 @@ def test_default_options_are_valid_and_settable(hip, precision):
     check(lib().lbm_device_free(0, buf))
 
 
-def test_large_fp32_domains_default_to_wide_workgroups(hip):
+@pytest.mark.parametrize("precision,small,large", [
+    ("f32", (1 | 16, 16), (64 | 1 | 16, 64 | 16)), ("f16s", (0, 0), (0, 256)),
+    ("bf16s", (0, 0), (0, 256)), ("f16c", (0, 0), (0, 0))])
+def test_large_domains_default_to_bigger_workgroups(hip, precision, small, large):
     """From 6 Mi cells (halo included) FP32 defaults to 512-thread
-    workgroups: beta always, alpha while no step pack is set."""
+    workgroups (beta always, alpha while no step pack is set) and F16S / BF16S
+    beta to 1024-thread workgroups."""
     import ctypes
     from cavityflow_cuda_amd._lib import check, lib
     v = ctypes.c_int()
-    for size, want in (((182, 182, 182), (1 | 16, 16)), ((183, 183, 183), (64 | 1 | 16, 64 | 16))):
-        s = hip.LbmSolverHIP(size, precision="f32", store_densities=False,
+    for size, want in (((182, 182, 182), small), ((183, 183, 183), large)):
+        s = hip.LbmSolverHIP(size, precision=precision, store_densities=False,
                              store_velocities=False)
         for k in (0, 1):
             check(lib().lbm_get_launch_options(s.handle, k, ctypes.byref(v)))
@@ -109,14 +113,17 @@ def test_large_fp32_domains_default_to_wide_workgroups(hip):
 @pytest.mark.parametrize("precision", ["f32", "f16s"])
 def test_box_steps_same_bits_under_wide_workgroups(hip, precision):
     """Sub-box steps (the exchange's inner and boundary boxes) map workgroup
-    cells through the box: 512-thread, two-cells-per-thread workgroups give
-    the bits of the 256-thread ones, partial last workgroup included."""
-    from cavityflow_cuda_amd._lib import LAUNCH_NONTEMPORAL, LAUNCH_WIDE, LAUNCH_WAVES2, check, lib
+    cells through the box: 512-thread, two-cells-per-thread and 1024-thread
+    workgroups give the bits of the 256-thread ones, partial last workgroup
+    included."""
+    from cavityflow_cuda_amd._lib import (LAUNCH_BIG, LAUNCH_NONTEMPORAL, LAUNCH_WIDE,
+                                          LAUNCH_WAVES2, check, lib)
     print("[case] wide box steps", precision, flush=True)
     boxes = [((1, 1, 1), SIZE), ((2, 3, 1), (33, 15, 11)), ((1, 1, 1), (37, 21, 1)),
              ((36, 2, 2), (2, 19, 11))]
     want = None
-    for opts in (0, LAUNCH_WIDE, LAUNCH_WIDE | LAUNCH_NONTEMPORAL | LAUNCH_WAVES2):
+    for opts in (0, LAUNCH_WIDE, LAUNCH_WIDE | LAUNCH_NONTEMPORAL | LAUNCH_WAVES2, LAUNCH_BIG,
+                 LAUNCH_BIG | LAUNCH_NONTEMPORAL | LAUNCH_WAVES2):
         s, full = state(hip, precision, np.random.default_rng(5))
         check(lib().lbm_set_launch_options(s.handle, 0, opts))
         check(lib().lbm_set_launch_options(s.handle, 1, opts & ~LAUNCH_NONTEMPORAL))
