@@ -155,10 +155,10 @@ hipError_t launch_reduce_boxes(const lbm_solver* const* s, const Box* b, int cou
 // concurrent plane streams), 16-bit storage needs every wave; F64 beta from
 // line-aligned accesses staged through LDS. FP32 from 512-thread workgroups
 // of two cells per thread on large domains (wide: at 192^3 and up; below
-// about 160^3 they leave too few workgroups), for alpha only when it does
-// not pack the x faces (with_pack: the fused pack stores cost those
-// workgroups more than they gain; FP32 alpha then keeps 3 waves per SIMD,
-// the best with the six-face exchange at 512^3). 16-bit storage: cached
+// about 160^3 they leave too few workgroups); with the fused x-face pack
+// (with_pack) FP32 beta on large domains runs in 1024-thread workgroups, one
+// per CU, and FP32 alpha on small domains keeps 3 waves per SIMD (measured
+// with the six-face exchange at 512^3). 16-bit storage: cached
 // accesses; F16S and BF16S beta in 1024-thread workgroups on large domains
 // (F16C beta, with more integer work per cell, loses with them).
 constexpr int64_t kWideCells = 6 << 20;   // halo-inclusive cells for `wide`
@@ -168,10 +168,12 @@ inline int default_launch(int precision, int kernel, bool with_pack, bool wide)
     switch (precision) {
     case LBM_F32:
         if (kernel == 0) {
-            if (with_pack)
+            if (with_pack && !wide)
                 return LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_WAVES3;
             return (wide ? LBM_LAUNCH_WIDE : 0) | LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_WAVES2;
         }
+        if (with_pack && wide)
+            return LBM_LAUNCH_BIG | LBM_LAUNCH_WAVES4;
         return (wide ? LBM_LAUNCH_WIDE : 0) | LBM_LAUNCH_WAVES2;
     case LBM_F64:
         return kernel == 0 ? (LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_WAVES2)

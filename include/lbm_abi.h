@@ -175,9 +175,9 @@ int lbm_simulation_step_beta(lbm_solver* solver, const int origin[3], const int 
 /* Launch variants of the step kernels; they change speed, never results.
  * kernel: 0 = alpha, 1 = beta. Defaults (measured on MI355X): F32 beta
  * WIDE | WAVES2 on domains of 6 Mi cells or more (halo included: from 183^3
- * interior cells) and WAVES2 below; F32 alpha NONTEMPORAL | WAVES3 with a step
- * pack (lbm_set_step_pack), else NONTEMPORAL | WAVES2, plus WIDE on those
- * large domains; F64 alpha NONTEMPORAL | WAVES2, beta LDS | WAVES2; F16S /
+ * interior cells), BIG | WAVES4 there with a step pack (lbm_set_step_pack),
+ * and WAVES2 below; F32 alpha NONTEMPORAL | WAVES2 plus WIDE on those large
+ * domains, NONTEMPORAL | WAVES3 with a step pack on smaller ones; F64 alpha NONTEMPORAL | WAVES2, beta LDS | WAVES2; F16S /
  * BF16S alpha none, beta BIG on those large domains (else none); F16C none.
  * Batches start from F32 / F64 alpha and beta as for 256-thread workgroups
  * with a step pack, 16-bit alpha NONTEMPORAL and beta none. No reference

@@ -96,8 +96,8 @@ def test_default_options_are_valid_and_settable(hip, precision):
     ("bf16s", (0, 0), (0, 256)), ("f16c", (0, 0), (0, 0))])
 def test_large_domains_default_to_bigger_workgroups(hip, precision, small, large):
     """From 6 Mi cells (halo included) FP32 defaults to 512-thread
-    workgroups (beta always, alpha while no step pack is set) and F16S / BF16S
-    beta to 1024-thread workgroups."""
+    workgroups and F16S / BF16S beta to 1024-thread workgroups; with a step
+    pack FP32 beta there takes 1024-thread workgroups at 4 waves per SIMD."""
     import ctypes
     from cavityflow_cuda_amd._lib import check, lib
     v = ctypes.c_int()
@@ -107,6 +107,15 @@ def test_large_domains_default_to_bigger_workgroups(hip, precision, small, large
         for k in (0, 1):
             check(lib().lbm_get_launch_options(s.handle, k, ctypes.byref(v)))
             assert v.value == want[k], (size, k)
+        if precision == "f32" and size[0] == 183:
+            buf = ctypes.c_void_p()
+            check(lib().lbm_device_alloc(0, s.face_elements(0) * 4, ctypes.byref(buf)))
+            check(lib().lbm_set_step_pack(s.handle, buf, buf))
+            for k, w in ((0, 64 | 1 | 16), (1, 256 | 4)):
+                check(lib().lbm_get_launch_options(s.handle, k, ctypes.byref(v)))
+                assert v.value == w, ("pack", k)
+            check(lib().lbm_set_step_pack(s.handle, None, None))
+            check(lib().lbm_device_free(0, buf))
         s.close()
 
 
