@@ -117,79 +117,88 @@ __global__ __launch_bounds__(kRedBlock) void k_reduce_boxes(ReduceBoxMembers mem
                         partials + (size_t)blockIdx.y * gridDim.x * kRedVals);
 }
 
-// Residual over FLUID cells. layout 0: slot l holds direction l (after beta);
-// layout 1: slot l holds direction opp(l) (after alpha). Block `block` of
-// `nblocks` strides over the cells; its sums go to partials[block].
-template <class P>
+// Residual over FLUID cells. LAYOUT 0: slot l holds direction l (after beta);
+// LAYOUT 1: slot l holds direction opp(l) (after alpha). A template argument:
+// with a run-time layout the slot permutation became register-indexed moves
+// that serialise the 19 loads. Block `block` of `nblocks` strides over the
+// cells; its sums go to partials[block].
+template <class P, int LAYOUT>
 __device__ __forceinline__ void residual_block(const typename P::S* __restrict__ dd,
                                                const uint8_t* __restrict__ flags,
                                                typename P::C* __restrict__ prev,
-                                               int64_t stride, uint32_t n, int layout,
+                                               int64_t stride, uint32_t n,
                                                uint32_t block, uint32_t nblocks,
                                                double* partials, const uint32_t* status)
 {
     using C = typename P::C;
     double acc[kRedVals] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (uint32_t id = block * kRedBlock + threadIdx.x; id < n; id += nblocks * kRedBlock) {
-        if (flags[id] != kFluid)
-            continue;
+        // every load of the cell is issued before the flag decides (one memory
+        // round trip per cell instead of two); cells other than FLUID add
+        // +0.0, which leaves the non-negative sums' bits as skipping them
+        // would, and store their snapshot back unchanged (no branch for the
+        // compiler to sink the snapshot loads into)
+        const bool fluid = flags[id] == kFluid;
         C d[kQ];
 #pragma unroll
         for (int l = 0; l < kQ; l++) {
-            const C v = P::load(dd[(int64_t)l * stride + id], l);
-            if (layout)
-                d[opp(l)] = v;
-            else
-                d[l] = v;
+            d[LAYOUT ? opp(l) : l] = P::load(dd[(int64_t)l * stride + id], l);
         }
+        const C p0 = prev[id], p1 = prev[stride + id], p2 = prev[2 * stride + id];
+        // empty statements that read every loaded value: the compiler cannot
+        // sink the loads below the flag test into the FLUID path
+#pragma unroll
+        for (int l = 0; l < kQ; l++)
+            asm volatile("" ::"v"(d[l]));
+        asm volatile("" ::"v"(p0), "v"(p1), "v"(p2));
         C rho, ux, uy, uz;
         moments(d, rho, ux, uy, uz);
-        const double dx = (double)ux - (double)prev[id];
-        const double dy = (double)uy - (double)prev[stride + id];
-        const double dz = (double)uz - (double)prev[2 * stride + id];
-        acc[0] += dx * dx + dy * dy + dz * dz;
-        acc[1] += (double)ux * ux + (double)uy * uy + (double)uz * uz;
-        acc[2] += 1.0;
-        prev[id] = ux;
-        prev[stride + id] = uy;
-        prev[2 * stride + id] = uz;
+        const double dx = (double)ux - (double)p0;
+        const double dy = (double)uy - (double)p1;
+        const double dz = (double)uz - (double)p2;
+        acc[0] += fluid ? dx * dx + dy * dy + dz * dz : 0.0;
+        acc[1] += fluid ? (double)ux * ux + (double)uy * uy + (double)uz * uz : 0.0;
+        acc[2] += fluid ? 1.0 : 0.0;
+        prev[id] = fluid ? ux : p0;
+        prev[stride + id] = fluid ? uy : p1;
+        prev[2 * stride + id] = fluid ? uz : p2;
     }
     if (poisoned(status))
         acc[0] = acc[1] = __builtin_nan("");
     block_store(acc, partials, block);
 }
 
-template <class P>
+template <class P, int LAYOUT>
 __global__ __launch_bounds__(kRedBlock) void k_residual(const typename P::S* __restrict__ dd,
                                                         const uint8_t* __restrict__ flags,
                                                         typename P::C* __restrict__ prev,
-                                                        int64_t stride, uint32_t n, int layout,
+                                                        int64_t stride, uint32_t n,
                                                         double* partials,
                                                         const uint32_t* status)
 {
-    residual_block<P>(dd, flags, prev, stride, n, layout, blockIdx.x, gridDim.x, partials,
-                      status);
+    residual_block<P, LAYOUT>(dd, flags, prev, stride, n, blockIdx.x, gridDim.x, partials,
+                              status);
 }
 
 // The residual of many solvers in one launch: grid.y = member, and member m's
 // blocks do exactly what k_residual's do for that solver alone, into their own
 // slice of partials, so each member's sums carry the same bits as
 // lbm_residual's.
-template <class P>
+template <class P, int LAYOUT>
 __global__ __launch_bounds__(kRedBlock) void k_residual_members(const ResidualMember* mem,
-                                                                int layout, double* partials)
+                                                                double* partials)
 {
     const ResidualMember& m = mem[blockIdx.y];
-    residual_block<P>((const typename P::S*)m.dd, m.flags, (typename P::C*)m.prev, m.stride,
-                      m.n, layout, blockIdx.x, gridDim.x,
-                      partials + (size_t)blockIdx.y * gridDim.x * kRedVals, m.status);
+    residual_block<P, LAYOUT>((const typename P::S*)m.dd, m.flags, (typename P::C*)m.prev,
+                              m.stride, m.n, blockIdx.x, gridDim.x,
+                              partials + (size_t)blockIdx.y * gridDim.x * kRedVals, m.status);
 }
 
 // Density and momentum of every cell of a box, computed from the populations
 // with the reference accumulation order; dst is dense [rho, jx, jy, jz][box].
-template <class P>
+template <class P, int LAYOUT>
 __global__ __launch_bounds__(kRedBlock) void k_moments(const typename P::S* __restrict__ dd,
-                                                       RedArgs a, int layout,
+                                                       RedArgs a,
                                                        typename P::C* __restrict__ dst)
 {
     using C = typename P::C;
@@ -204,13 +213,8 @@ __global__ __launch_bounds__(kRedBlock) void k_moments(const typename P::S* __re
                                    a.ox + (int32_t)x);
     C d[kQ];
 #pragma unroll
-    for (int l = 0; l < kQ; l++) {
-        const C v = P::load(dd[(int64_t)l * a.stride + id], l);
-        if (layout)
-            d[opp(l)] = v;
-        else
-            d[l] = v;
-    }
+    for (int l = 0; l < kQ; l++)
+        d[LAYOUT ? opp(l) : l] = P::load(dd[(int64_t)l * a.stride + id], l);
     C rho, ux, uy, uz;
     moments(d, rho, ux, uy, uz);
     if (poisoned(a.status))
@@ -272,10 +276,16 @@ template <class P>
 static hipError_t residual_t(const lbm_solver* s, int layout, double* partials, double* out,
                              hipStream_t st)
 {
-    hipLaunchKernelGGL(k_residual<P>, dim3(kReduceBlocks), dim3(kRedBlock), 0, st,
-                       (const typename P::S*)s->dd, s->flags,
-                       (typename P::C*)s->residual_prev, s->stride, (uint32_t)s->n, layout,
-                       partials, s->status);
+    if (layout)
+        hipLaunchKernelGGL((k_residual<P, 1>), dim3(kReduceBlocks), dim3(kRedBlock), 0, st,
+                           (const typename P::S*)s->dd, s->flags,
+                           (typename P::C*)s->residual_prev, s->stride, (uint32_t)s->n,
+                           partials, s->status);
+    else
+        hipLaunchKernelGGL((k_residual<P, 0>), dim3(kReduceBlocks), dim3(kRedBlock), 0, st,
+                           (const typename P::S*)s->dd, s->flags,
+                           (typename P::C*)s->residual_prev, s->stride, (uint32_t)s->n,
+                           partials, s->status);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, st, partials, kReduceBlocks, out);
     return hipGetLastError();
 }
@@ -285,9 +295,13 @@ static hipError_t moments_t(const lbm_solver* s, const Box& b, int layout, void*
                             hipStream_t st)
 {
     const RedArgs a = red_args(s, b);
-    hipLaunchKernelGGL(k_moments<P>, dim3((a.cells + kRedBlock - 1) / kRedBlock),
-                       dim3(kRedBlock), 0, st, (const typename P::S*)s->dd, a, layout,
-                       (typename P::C*)dst);
+    const dim3 grid((a.cells + kRedBlock - 1) / kRedBlock);
+    if (layout)
+        hipLaunchKernelGGL((k_moments<P, 1>), grid, dim3(kRedBlock), 0, st,
+                           (const typename P::S*)s->dd, a, (typename P::C*)dst);
+    else
+        hipLaunchKernelGGL((k_moments<P, 0>), grid, dim3(kRedBlock), 0, st,
+                           (const typename P::S*)s->dd, a, (typename P::C*)dst);
     return hipGetLastError();
 }
 
@@ -321,8 +335,12 @@ template <class P>
 static hipError_t residual_members_t(const ResidualMember* mem, int count, int layout,
                                      double* partials, double* out, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_residual_members<P>, dim3(kReduceBlocks, count), dim3(kRedBlock), 0,
-                       st, mem, layout, partials);
+    if (layout)
+        hipLaunchKernelGGL((k_residual_members<P, 1>), dim3(kReduceBlocks, count),
+                           dim3(kRedBlock), 0, st, mem, partials);
+    else
+        hipLaunchKernelGGL((k_residual_members<P, 0>), dim3(kReduceBlocks, count),
+                           dim3(kRedBlock), 0, st, mem, partials);
     hipLaunchKernelGGL(k_finalize, dim3(count), dim3(64), 0, st, partials, kReduceBlocks, out);
     return hipGetLastError();
 }
