@@ -174,15 +174,16 @@ def cpu_baseline(size, params, sample_steps, threads_all):
                              store_densities=False, store_velocities=False,
                              nthreads=threads_all)
         o.set_driven_cavity()
-        # a sample of about 10 s: as many whole steps as the single-thread
-        # rate suggests, at least the single-thread sample
-        steps_all = max(sample_steps, 2 * int(5.0 * threads_all / max(dt / sample_steps, 1e-3)))
-        steps_all = min(steps_all, 400)
+        # a sample of about 10 s: whole steps (alpha + beta) until 10 s have
+        # passed, at least the single-thread sample, at most 400 half-steps
         t0 = time.perf_counter()
-        for n in range(steps_all):
-            o.beta() if n & 1 else o.alpha()
+        n = 0
+        while n < 400 and (n < sample_steps or time.perf_counter() - t0 < 10.0):
+            o.alpha()
+            o.beta()
+            n += 2
         dt = time.perf_counter() - t0
-        sample_steps = steps_all
+        sample_steps = n
         out["all"] = dict(value=round(cells * sample_steps / dt / 1e6, 3), unit="MLUPS",
                           cores=threads_all, kind="port", **host,
                           sample=f"{sample_steps} steps, oracle C restatement, OpenMP over z, "
