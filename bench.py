@@ -3,9 +3,10 @@
 Default workload: the lid-driven cavity at one rank's real subdomain of a
 2x2x2 decomposition -- 512^3 interior cells (514^3 with the wall/halo layer,
 10.3 GB of FP32 populations), Re 1000, lid velocity 0.1, FP32 -- with all six
-face halos packed (CComm shapes, 19 x 514 x 514) and unpacked on the device
-after every half-step. The unpack targets a shadow subdomain of the same shape
-so the cavity walls stay physical.
+face halos packed (CComm face shapes, 514 x 514 cells; the 5 populations per face
+and phase that the receiving interior reads) and unpacked on the device after
+every half-step. The unpack targets a shadow subdomain of the same shape so the
+cavity walls stay physical.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 512]
                     [--precision f32|f64|f16s|bf16s|f16c] [--no-halo]
@@ -513,21 +514,28 @@ def step_run(args, cf, L, hip, stream, precision, steps, warmup):
                     check(L.lbm_set_density_distributions_dir(shadow.handle, o, z, r, hbufs[d],
                                                               0, stream))
 
+    yz_pairs = args.yz_copies == "pair"
+
     def halo_exchange(phase):
         if host_ex:
             return host_halo_exchange(phase)
         # reference order: x faces, then y, then z (CController.cpp:369-570)
         for axis in range(3):
+            pair = yz_pairs if axis else not separate_unpack
             if not (fused and axis == 0):
-                for d in (2 * axis, 2 * axis + 1):
-                    check(L.lbm_pack_face(solver.handle, d, phase, bufs[d], stream))
-            if separate_unpack:
-                for d in (2 * axis, 2 * axis + 1):
-                    check(L.lbm_unpack_face(shadow.handle, d ^ 1, phase, bufs[d], stream))
-            else:
+                if axis and yz_pairs:
+                    check(L.lbm_pack_face_pair(solver.handle, axis, phase, bufs[2 * axis],
+                                               bufs[2 * axis + 1], stream))
+                else:
+                    for d in (2 * axis, 2 * axis + 1):
+                        check(L.lbm_pack_face(solver.handle, d, phase, bufs[d], stream))
+            if pair:
                 # the low face receives what the high face sent, and vice versa
                 check(L.lbm_unpack_face_pair(shadow.handle, axis, phase, bufs[2 * axis + 1],
                                              bufs[2 * axis], stream))
+            else:
+                for d in (2 * axis, 2 * axis + 1):
+                    check(L.lbm_unpack_face(shadow.handle, d ^ 1, phase, bufs[d], stream))
 
     step_fns = (L.lbm_simulation_step_alpha, L.lbm_simulation_step_beta)
     counter = [0]
@@ -774,6 +782,9 @@ def main():
     ap.add_argument("--unpack", choices=("auto", "pair", "separate"), default="auto",
                     help="x-face unpack: both faces in one call (pair) or one call each; "
                          "auto = pair for 16-bit storage, separate otherwise")
+    ap.add_argument("--yz-copies", choices=("pair", "separate"), default="pair",
+                    help="y / z faces: both faces of an axis packed and unpacked in one "
+                         "launch each (pair) or one launch per face")
     ap.add_argument("--host-exchange", action="store_true",
                     help="step: faces pass through pinned host buffers with the reference's "
                          "calls (getDensityDistributions / setDensityDistributions), "
@@ -907,6 +918,7 @@ def main():
             "precision": args.precision,
             "halo_exchange": halo,
             "fused_x_pack": fused, "paired_unpack": halo and not separate_unpack and not host_ex,
+            "paired_yz_copies": halo and not host_ex and args.yz_copies == "pair",
             "host_exchange": host_ex,
             "parallelism": "single GPU (one rank's subdomain)",
             "launch_options": {"alpha": launch_opts[0], "beta": launch_opts[1]},

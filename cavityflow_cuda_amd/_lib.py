@@ -32,7 +32,7 @@ EXPORTED = (
     "lbm_set_density_distributions_dir", "lbm_get_flags", "lbm_set_flags",
     "lbm_get_velocities", "lbm_set_velocities", "lbm_get_densities", "lbm_set_densities",
     "lbm_face_box", "lbm_face_elements", "lbm_pack_face", "lbm_unpack_face",
-    "lbm_unpack_face_pair",
+    "lbm_unpack_face_pair", "lbm_pack_face_pair",
     "lbm_exchange_face", "lbm_device_alloc", "lbm_device_free",
     "lbm_set_launch_options", "lbm_get_launch_options", "lbm_set_step_pack", "lbm_get_moments",
     "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_size", "lbm_batch_set_tuning",
@@ -156,6 +156,7 @@ def lib() -> ctypes.CDLL:
         "lbm_pack_face": ([P, I, I, P, P], I),
         "lbm_unpack_face": ([P, I, I, P, P], I),
         "lbm_unpack_face_pair": ([P, I, I, P, P, P], I),
+        "lbm_pack_face_pair": ([P, I, I, P, P, P], I),
         "lbm_exchange_face": ([P, I, I, P, P], I),
         "lbm_device_alloc": ([I, ctypes.c_size_t, ctypes.POINTER(P)], I),
         "lbm_device_free": ([I, P], I),

@@ -768,9 +768,22 @@ int lbm_unpack_face_pair(lbm_solver* s, int axis, int phase, const void* lo_buf,
     }
     int rc;
     if (axis != 0) {
-        if ((rc = lbm_unpack_face(s, 2 * axis, phase, lo_buf, stream)))
+        // y / z: both faces' boxes in one launch (disjoint planes of the array,
+        // or one plane with disjoint lattice vectors when the axis is one cell)
+        Box b[2];
+        uint32_t mask[2];
+        for (int k = 0; k < 2; k++) {
+            int o[3], z[3];
+            if ((rc = lbm_face_box(s, 2 * axis + k, phase == 0 ? 1 : 0, o, z)))
+                return rc;
+            b[k] = Box{o[0], o[1], o[2], z[0], z[1], z[2]};
+            mask[k] = exchange_mask((2 * axis + k) ^ 1, phase);
+        }
+        if ((rc = set_device(s)))
             return rc;
-        return lbm_unpack_face(s, 2 * axis + 1, phase, hi_buf, stream);
+        const void* src[2] = {lo_buf, hi_buf};
+        LBM_HIP(launch_dd_set_pair(s, b, src, mask, stream), "pair unpack kernel");
+        return LBM_OK;
     }
     if ((rc = set_device(s)))
         return rc;
@@ -783,6 +796,30 @@ int lbm_unpack_face_pair(lbm_solver* s, int axis, int phase, const void* lo_buf,
     const uint32_t mask_hi = exchange_mask(LBM_LEFT, phase);
     LBM_HIP(launch_set_dd_xpair(s, lo_buf, hi_buf, x_lo, x_hi, mask_lo, mask_hi, true, stream),
             "pair unpack kernel");
+    return LBM_OK;
+}
+
+int lbm_pack_face_pair(lbm_solver* s, int axis, int phase, void* lo_buf, void* hi_buf,
+                       hipStream_t stream)
+{
+    if (!s || axis < 0 || axis > 2 || phase < 0 || phase > 1 || !lo_buf || !hi_buf) {
+        set_error("lbm_pack_face_pair: invalid argument");
+        return LBM_ERR_ARG;
+    }
+    int rc;
+    Box b[2];
+    uint32_t mask[2];
+    for (int k = 0; k < 2; k++) {
+        int o[3], z[3];
+        if ((rc = lbm_face_box(s, 2 * axis + k, phase == 0 ? 0 : 1, o, z)))
+            return rc;
+        b[k] = Box{o[0], o[1], o[2], z[0], z[1], z[2]};
+        mask[k] = exchange_mask(2 * axis + k, phase);
+    }
+    if ((rc = set_device(s)))
+        return rc;
+    void* dst[2] = {lo_buf, hi_buf};
+    LBM_HIP(launch_dd_get_pair(s, b, dst, mask, stream), "pair pack kernel");
     return LBM_OK;
 }
 

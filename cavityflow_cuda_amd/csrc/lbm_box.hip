@@ -33,19 +33,29 @@ __device__ __forceinline__ uint32_t box_index(const BoxArgs& a, uint32_t gid)
     return (uint32_t)((a.oz + (int32_t)z) * a.nxy + (a.oy + (int32_t)y) * a.nx + a.ox + (int32_t)x);
 }
 
-// populations: grid.y = index into the plane list; shifted storage is
+// One population copy of up to two boxes in one launch (grid.z = box): the two
+// faces of an exchange axis, each with its own plane list and buffer.
+struct BoxCopy {
+    BoxArgs box[2];
+    void* buf[2];
+};
+
+// populations: grid.y = index into the box's plane list; shifted storage is
 // converted to f. NT: streaming accesses, for boxes one cell thick on some axis
 // (the faces of a halo exchange), which the next step does not reuse from the
 // caches.
 template <class P, bool NT>
 __global__ __launch_bounds__(kCopyBlock) void k_get_dd(const typename P::S* __restrict__ dd,
-                                                       typename P::C* __restrict__ dst,
-                                                       BoxArgs a)
+                                                       BoxCopy c)
 {
+    const BoxArgs& a = c.box[blockIdx.z];
+    if ((int)blockIdx.y >= a.nplanes)
+        return;
     const int l = a.plane[blockIdx.y];
     const uint32_t gid = blockIdx.x * kCopyBlock + threadIdx.x;
     if (gid >= a.cells)
         return;
+    typename P::C* __restrict__ dst = (typename P::C*)c.buf[blockIdx.z];
     const uint32_t id = box_index(a, gid);
     const typename P::S v = NT ? __builtin_nontemporal_load(dd + (int64_t)l * a.stride + id)
                                : dd[(int64_t)l * a.stride + id];
@@ -54,13 +64,16 @@ __global__ __launch_bounds__(kCopyBlock) void k_get_dd(const typename P::S* __re
 
 template <class P, bool NT>
 __global__ __launch_bounds__(kCopyBlock) void k_set_dd(typename P::S* __restrict__ dd,
-                                                       const typename P::C* __restrict__ src,
-                                                       BoxArgs a)
+                                                       BoxCopy c)
 {
+    const BoxArgs& a = c.box[blockIdx.z];
+    if ((int)blockIdx.y >= a.nplanes)
+        return;
     const int l = a.plane[blockIdx.y];
     const uint32_t gid = blockIdx.x * kCopyBlock + threadIdx.x;
     if (gid >= a.cells)
         return;
+    const typename P::C* __restrict__ src = (const typename P::C*)c.buf[blockIdx.z];
     const uint32_t id = box_index(a, gid);
     uint32_t ov = 0;
     const typename P::S v = P::store_checked(src[(int64_t)l * a.cells + gid], l, ov);
@@ -184,38 +197,66 @@ static BoxArgs box_args(const lbm_solver* s, const Box& b, uint32_t mask)
     return a;
 }
 
-template <class P>
-static hipError_t get_dd_t(const lbm_solver* s, const Box& b, void* dst, uint32_t mask,
-                           hipStream_t st)
+// The population copy of `nbox` boxes (1 or 2) in one launch; boxes one cell
+// thick on some axis stream.
+template <class P, bool GET>
+static hipError_t copy_dd_t(const lbm_solver* s, const Box* b, void* const* buf,
+                            const uint32_t* mask, int nbox, hipStream_t st)
 {
-    BoxArgs a = box_args(s, b, mask);
-    if (!a.nplanes)
+    BoxCopy c{};
+    uint32_t cells = 0;
+    int nplanes = 0;
+    bool thin = true;
+    for (int k = 0; k < nbox; k++) {
+        c.box[k] = box_args(s, b[k], mask[k]);
+        c.buf[k] = buf[k];
+        cells = c.box[k].cells > cells ? c.box[k].cells : cells;
+        nplanes = c.box[k].nplanes > nplanes ? c.box[k].nplanes : nplanes;
+        thin = thin && (b[k].bx == 1 || b[k].by == 1 || b[k].bz == 1);
+    }
+    if (!nplanes || !cells)
         return hipSuccess;
-    dim3 grid((a.cells + kCopyBlock - 1) / kCopyBlock, a.nplanes);
-    if (b.bx == 1 || b.by == 1 || b.bz == 1)
-        hipLaunchKernelGGL((k_get_dd<P, true>), grid, dim3(kCopyBlock), 0, st,
-                           (const typename P::S*)s->dd, (typename P::C*)dst, a);
-    else
-        hipLaunchKernelGGL((k_get_dd<P, false>), grid, dim3(kCopyBlock), 0, st,
-                           (const typename P::S*)s->dd, (typename P::C*)dst, a);
+    const dim3 grid((cells + kCopyBlock - 1) / kCopyBlock, nplanes, nbox);
+    if (GET) {
+        const auto* dd = (const typename P::S*)s->dd;
+        if (thin)
+            hipLaunchKernelGGL((k_get_dd<P, true>), grid, dim3(kCopyBlock), 0, st, dd, c);
+        else
+            hipLaunchKernelGGL((k_get_dd<P, false>), grid, dim3(kCopyBlock), 0, st, dd, c);
+    } else {
+        auto* dd = (typename P::S*)s->dd;
+        if (thin)
+            hipLaunchKernelGGL((k_set_dd<P, true>), grid, dim3(kCopyBlock), 0, st, dd, c);
+        else
+            hipLaunchKernelGGL((k_set_dd<P, false>), grid, dim3(kCopyBlock), 0, st, dd, c);
+    }
     return hipGetLastError();
 }
 
-template <class P>
-static hipError_t set_dd_t(const lbm_solver* s, const Box& b, const void* src, uint32_t mask,
-                           hipStream_t st)
+template <bool GET>
+static hipError_t copy_dd(const lbm_solver* s, const Box* b, void* const* buf,
+                          const uint32_t* mask, int nbox, hipStream_t st)
 {
-    BoxArgs a = box_args(s, b, mask);
-    if (!a.nplanes)
-        return hipSuccess;
-    dim3 grid((a.cells + kCopyBlock - 1) / kCopyBlock, a.nplanes);
-    if (b.bx == 1 || b.by == 1 || b.bz == 1)
-        hipLaunchKernelGGL((k_set_dd<P, true>), grid, dim3(kCopyBlock), 0, st,
-                           (typename P::S*)s->dd, (const typename P::C*)src, a);
-    else
-        hipLaunchKernelGGL((k_set_dd<P, false>), grid, dim3(kCopyBlock), 0, st,
-                           (typename P::S*)s->dd, (const typename P::C*)src, a);
-    return hipGetLastError();
+    switch (s->p.precision) {
+    case LBM_F32: return copy_dd_t<StoreF32, GET>(s, b, buf, mask, nbox, st);
+    case LBM_F64: return copy_dd_t<StoreF64, GET>(s, b, buf, mask, nbox, st);
+    case LBM_F16S: return copy_dd_t<StoreF16S, GET>(s, b, buf, mask, nbox, st);
+    case LBM_BF16S: return copy_dd_t<StoreBF16S, GET>(s, b, buf, mask, nbox, st);
+    case LBM_F16C: return copy_dd_t<StoreF16C, GET>(s, b, buf, mask, nbox, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_dd_get_pair(const lbm_solver* s, const Box* b, void* const* dst,
+                              const uint32_t* mask, hipStream_t st)
+{
+    return copy_dd<true>(s, b, dst, mask, 2, st);
+}
+
+hipError_t launch_dd_set_pair(const lbm_solver* s, const Box* b, const void* const* src,
+                              const uint32_t* mask, hipStream_t st)
+{
+    return copy_dd<false>(s, b, const_cast<void* const*>(src), mask, 2, st);
 }
 
 template <class C>
@@ -236,16 +277,8 @@ static hipError_t field_t(const lbm_solver* s, int field, const Box& b, void* bu
 hipError_t launch_box_get(const lbm_solver* s, int field, const Box& b, void* dst,
                           uint32_t mask, hipStream_t st)
 {
-    if (field == 0) {
-        switch (s->p.precision) {
-        case LBM_F32: return get_dd_t<StoreF32>(s, b, dst, mask, st);
-        case LBM_F64: return get_dd_t<StoreF64>(s, b, dst, mask, st);
-        case LBM_F16S: return get_dd_t<StoreF16S>(s, b, dst, mask, st);
-        case LBM_BF16S: return get_dd_t<StoreBF16S>(s, b, dst, mask, st);
-        case LBM_F16C: return get_dd_t<StoreF16C>(s, b, dst, mask, st);
-        default: return hipErrorInvalidValue;
-        }
-    }
+    if (field == 0)
+        return copy_dd<true>(s, &b, &dst, &mask, 1, st);
     if (s->p.precision == LBM_F64)
         return field_t<double>(s, field, b, dst, true, st);
     return field_t<float>(s, field, b, dst, true, st);
@@ -255,14 +288,8 @@ hipError_t launch_box_set(const lbm_solver* s, int field, const Box& b, const vo
                           uint32_t mask, hipStream_t st)
 {
     if (field == 0) {
-        switch (s->p.precision) {
-        case LBM_F32: return set_dd_t<StoreF32>(s, b, src, mask, st);
-        case LBM_F64: return set_dd_t<StoreF64>(s, b, src, mask, st);
-        case LBM_F16S: return set_dd_t<StoreF16S>(s, b, src, mask, st);
-        case LBM_BF16S: return set_dd_t<StoreBF16S>(s, b, src, mask, st);
-        case LBM_F16C: return set_dd_t<StoreF16C>(s, b, src, mask, st);
-        default: return hipErrorInvalidValue;
-        }
+        void* buf = const_cast<void*>(src);
+        return copy_dd<false>(s, &b, &buf, &mask, 1, st);
     }
     if (s->p.precision == LBM_F64)
         return field_t<double>(s, field, b, const_cast<void*>(src), false, st);

@@ -107,6 +107,12 @@ hipError_t launch_beta(const lbm_solver* s, const Box& b, hipStream_t st);
 // planes), 1 velocities (3), 2 densities (1)
 hipError_t launch_box_get(const lbm_solver* s, int field, const Box& b, void* dst,
                           uint32_t mask, hipStream_t st);
+// the population copies of two boxes (the faces of one axis) in one launch,
+// each with its own buffer and plane mask
+hipError_t launch_dd_get_pair(const lbm_solver* s, const Box* b, void* const* dst,
+                              const uint32_t* mask, hipStream_t st);
+hipError_t launch_dd_set_pair(const lbm_solver* s, const Box* b, const void* const* src,
+                              const uint32_t* mask, hipStream_t st);
 // both x faces in one launch (nt: streaming stores)
 hipError_t launch_set_dd_xpair(const lbm_solver* s, const void* lo, const void* hi, int x_lo,
                                int x_hi, uint32_t mask_lo, uint32_t mask_hi, bool nt,

@@ -258,6 +258,15 @@ class LbmSolverHIP:
             raise LbmError(_lib.LBM_OK - 1, "unpack_face needs a device buffer")
         check(lib().lbm_unpack_face(self._h, int(direction), int(phase), ptr, _stream(stream)))
 
+    def pack_face_pair(self, axis, phase, lo_buf, hi_buf, stream=None):
+        """Both faces of one axis in one call: the low face (2 * axis) into
+        lo_buf, the high face (2 * axis + 1) into hi_buf."""
+        lo, lo_dev, keep_lo = _buffer(lo_buf)
+        hi, hi_dev, keep_hi = _buffer(hi_buf)
+        if not (lo_dev and hi_dev):
+            raise LbmError(_lib.LBM_OK - 1, "pack_face_pair needs device buffers")
+        check(lib().lbm_pack_face_pair(self._h, int(axis), int(phase), lo, hi, _stream(stream)))
+
     def unpack_face_pair(self, axis, phase, lo_buf, hi_buf, stream=None):
         """Both faces of one axis in one call: lo_buf into the low face
         (2 * axis), hi_buf into the high face (2 * axis + 1)."""

@@ -299,15 +299,21 @@ int lbm_pack_face(lbm_solver* solver, int direction, int phase, void* dev_buf,
 int lbm_unpack_face(lbm_solver* solver, int direction, int phase, const void* dev_buf,
                     hipStream_t stream);
 
-/* Both faces of one axis (0 = x, 1 = y, 2 = z): the same result as
- * lbm_unpack_face(2 * axis, phase, lo_buf) followed by
- * lbm_unpack_face(2 * axis + 1, phase, hi_buf). For x, one launch writes the
- * two faces row by row, so the halo cells at the end of one row and the
- * start of the next (adjacent in memory) are written together. No reference
- * counterpart: the reference unpacks each face with its own
- * setDensityDistributions call (CController.cpp:411-435, 849-874). */
+/* Both faces of one axis (0 = x, 1 = y, 2 = z) in one launch: the same result
+ * as lbm_unpack_face(2 * axis, phase, lo_buf) followed by
+ * lbm_unpack_face(2 * axis + 1, phase, hi_buf). For x, the launch writes the
+ * two faces row by row. No reference counterpart: the reference unpacks each
+ * face with its own setDensityDistributions call (CController.cpp:411-435,
+ * 849-874). */
 int lbm_unpack_face_pair(lbm_solver* solver, int axis, int phase, const void* lo_buf,
                          const void* hi_buf, hipStream_t stream);
+
+/* Both faces of one axis packed in one launch: the same buffers as
+ * lbm_pack_face(2 * axis, phase, lo_buf) and lbm_pack_face(2 * axis + 1,
+ * phase, hi_buf). No reference counterpart: the reference packs each face with
+ * its own getDensityDistributions call (CController.cpp:369-409, 807-847). */
+int lbm_pack_face_pair(lbm_solver* solver, int axis, int phase, void* lo_buf, void* hi_buf,
+                       hipStream_t stream);
 
 /* Fused halo packing of the two x faces into caller-owned device buffers of
  * lbm_face_elements(LEFT) elements each (NULL disables a side, both NULL

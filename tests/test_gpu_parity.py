@@ -287,11 +287,52 @@ def test_pack_unpack_faces_match_oracle(hip, oracle_cpu):
 
 
 @pytest.mark.parametrize("prec", ["f32", "f64", "f16s", "bf16s", "f16c"])
-@pytest.mark.parametrize("size", [(9, 6, 7), (1, 4, 5)])
+@pytest.mark.parametrize("size", [(9, 6, 7), (1, 4, 5), (5, 1, 1)])
+def test_pack_face_pair_equals_two_packs(hip, prec, size):
+    """lbm_pack_face_pair(axis) fills both buffers exactly as the two
+    single-face packs do, planes outside each face's 5 untouched, for every
+    axis and phase."""
+    from hiprt import DeviceBuffer
+    say("pack_face_pair", prec, size)
+    rng = np.random.default_rng(37)
+    dt = np.float64 if prec == "f64" else np.float32
+    s = hip.LbmSolverHIP(size, (OBST,) * 6, prec)
+    o, z = full_box(s)
+    cells = z[0] * z[1] * z[2]
+    w = np.array([1 / 18] * 4 + [1 / 36] * 12 + [1 / 18] * 2 + [1 / 3])
+    s.setDensityDistributions((w[:, None] * (1 + 0.1 * rng.standard_normal((19, cells))))
+                              .astype(dt).ravel(), o, z)
+    for axis in range(3):
+        n = s.face_elements(2 * axis)
+        for phase in (0, 1):
+            bufs = [DeviceBuffer(n * np.dtype(dt).itemsize) for _ in range(4)]
+            for b in bufs:
+                b.upload(np.full(n, -7.0, dt))
+            s.pack_face(2 * axis, phase, bufs[0])
+            s.pack_face(2 * axis + 1, phase, bufs[1])
+            s.pack_face_pair(axis, phase, bufs[2], bufs[3])
+            s.sync()
+            got = [b.download(dt, n).tobytes() for b in bufs]
+            for b in bufs:
+                b.free()
+            assert got[2] == got[0] and got[3] == got[1], (axis, phase)
+    from cavityflow_cuda_amd import LbmError
+    buf = DeviceBuffer(64)
+    with pytest.raises(LbmError):
+        s.pack_face_pair(3, 0, buf, buf)
+    with pytest.raises(LbmError):
+        s.pack_face_pair(1, 2, buf, buf)
+    buf.free()
+    s.close()
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64", "f16s", "bf16s", "f16c"])
+@pytest.mark.parametrize("size", [(9, 6, 7), (1, 4, 5), (5, 1, 1)])
 def test_unpack_face_pair_equals_two_unpacks(hip, prec, size):
     """lbm_unpack_face_pair(axis) leaves exactly the state of the two
-    single-face unpacks, for every axis and phase (x: the one-launch pair
-    kernel; the 3-wide x extent puts both phase-1 planes in one column)."""
+    single-face unpacks, for every axis and phase (one launch per axis; a
+    one-cell axis puts both phase-1 planes in one plane of the array, with
+    disjoint lattice vectors)."""
     from hiprt import DeviceBuffer
     say("unpack_face_pair", prec, size)
     rng = np.random.default_rng(31)
