@@ -124,7 +124,6 @@ template <class P>
 __device__ __forceinline__ void xpack_alpha(const StepArgs<typename P::C>& a, uint32_t row,
                                             bool pl, bool pr, const typename P::C (&v)[kQ])
 {
-    using C = typename P::C;
 #pragma unroll
     for (int s = 0; s < kQ - 1; s++) {
         if (cx(s) == 0 || !(cx(s) > 0 ? pl : pr))
