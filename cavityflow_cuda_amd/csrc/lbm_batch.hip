@@ -17,6 +17,7 @@
 #include <cmath>
 #include <limits>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "lbm_abi.h"
@@ -235,6 +236,10 @@ int upload_tiles(lbm_batch* b, uint32_t tile_cells, std::vector<MemberDesc>& des
     return LBM_OK;
 }
 
+// candidates besides the starting tuning that the autotuner's final round
+// times again over longer runs
+constexpr size_t kFinalists = 4;
+
 constexpr int kBatchOptions =
     LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_XCD_REMAP | LBM_LAUNCH_WAVES4 | LBM_LAUNCH_WAVES3 |
     LBM_LAUNCH_WAVES2;
@@ -263,9 +268,17 @@ size_t waves_lds(int opts, int block)
     return (size_t)(160 * 1024) / (size_t)(per_cu + 1) + 1024;
 }
 
+bool same_tuning(const lbm_batch_tuning& a, const lbm_batch_tuning& b)
+{
+    return a.block == b.block && a.cells_per_thread == b.cells_per_thread &&
+           a.waves_per_cu == b.waves_per_cu && a.options[0] == b.options[0] &&
+           a.options[1] == b.options[1] && a.pairs == b.pairs && a.group_mb == b.group_mb;
+}
+
 bool valid_tuning(const lbm_batch_tuning& t)
 {
-    return t.pairs >= 0 && t.pairs <= kPairShapes && (t.block == 256 || t.block == 512) &&
+    return t.pairs >= 0 && t.pairs <= kPairShapes &&
+           (t.block == 256 || t.block == 512 || t.block == 1024) &&
            t.group_mb >= 0 && !(t.pairs && t.group_mb) &&
            (t.cells_per_thread == 1 || t.cells_per_thread == 2 || t.cells_per_thread == 4) &&
            t.waves_per_cu >= 0 && t.waves_per_cu <= 64 &&
@@ -367,6 +380,11 @@ int prepare_pairs(lbm_batch* b, int pairs)
 
 int apply_tuning(lbm_batch* b, const lbm_batch_tuning& t)
 {
+    if (t.block == 1024 && b->precision == LBM_F64) {
+        // 1024 threads leave 128 VGPRs per thread: the FP64 kernels would spill
+        set_error("batch tuning: FP64 members take workgroups of 256 or 512 threads");
+        return LBM_ERR_UNSUPPORTED;
+    }
     drop_graphs(b);
     if (t.pairs) {
         int rc = prepare_pairs(b, t.pairs);
@@ -472,6 +490,9 @@ template <class P>
 hipError_t launch_tiles_p(const lbm_batch* b, uint32_t lo, uint32_t hi, int phase,
                           hipStream_t st)
 {
+    if constexpr (!std::is_same_v<P, StoreF64>)
+        if (b->tuning.block == 1024)
+            return launch_tiles_t<P, 1024>(b, lo, hi, phase, st);
     return b->tuning.block == 512 ? launch_tiles_t<P, 512>(b, lo, hi, phase, st)
                                   : launch_tiles_t<P, 256>(b, lo, hi, phase, st);
 }
@@ -780,11 +801,13 @@ int lbm_batch_autotune(lbm_batch* b, int trial_steps, hipStream_t stream,
     lbm_batch_tuning best = b->tuning;
     best.pairs = 0;
     float best_step = std::numeric_limits<float>::infinity();
+    std::vector<std::pair<float, lbm_batch_tuning>> timed;   // every trial, for the final round
     auto consider = [&](const lbm_batch_tuning& c) -> int {
         float t = 0.f;
         const int r = step_time(c, trial_steps, &t);
         if (r)
             return r;
+        timed.push_back({t, c});
         if (t < best_step) {
             best_step = t;
             best = c;
@@ -792,9 +815,14 @@ int lbm_batch_autotune(lbm_batch* b, int trial_steps, hipStream_t stream,
         return LBM_OK;
     };
     const lbm_batch_tuning start = best;
-    for (int blk : {256, 512})
+    // 1024-thread workgroups: not for FP64 (register spills), and at most two
+    // are resident per CU
+    const int max_block = b->precision == LBM_F64 ? 512 : 1024;
+    for (int blk : {256, 512, 1024})
         for (int cpt : {1, 2, 4})
             for (int wpc : {0, 1, 2, 3, 4, 8, 16}) {
+                if (blk > max_block || (blk == 1024 && wpc > 2))
+                    continue;
                 lbm_batch_tuning cand = start;
                 cand.block = blk;
                 cand.cells_per_thread = cpt;
@@ -826,8 +854,10 @@ int lbm_batch_autotune(lbm_batch* b, int trial_steps, hipStream_t stream,
     // differ from the streaming one: the shapes once more, grouped
     if (best.group_mb > 0) {
         const lbm_batch_tuning grouped = best;
-        for (int blk : {256, 512})
+        for (int blk : {256, 512, 1024})
             for (int cpt : {1, 2, 4}) {
+                if (blk > max_block || (blk == 1024 && grouped.waves_per_cu > 2))
+                    continue;
                 lbm_batch_tuning cand = grouped;
                 cand.block = blk;
                 cand.cells_per_thread = cpt;
@@ -850,31 +880,43 @@ int lbm_batch_autotune(lbm_batch* b, int trial_steps, hipStream_t stream,
             float t = 0.f;
             if ((rc = step_time(cand, trial_steps, &t)))
                 return rc;
+            timed.push_back({t, cand});
             if (t < best_step) {
                 best_step = t;
                 best = cand;
             }
         }
-    // short trials are noisy: the winner must also beat the starting tuning
-    // over longer, interleaved runs
-    if (best.pairs != start.pairs || best.group_mb != start.group_mb ||
-        best.block != start.block ||
-        best.cells_per_thread != start.cells_per_thread ||
-        best.waves_per_cu != start.waves_per_cu || best.options[0] != start.options[0] ||
-        best.options[1] != start.options[1]) {
+    // Short trials are noisy, and the more candidates, the likelier a lucky
+    // trial wins: the few fastest distinct candidates and the starting tuning
+    // run again over longer runs, interleaved, and the least total time wins
+    // (the starting tuning on a tie).
+    std::stable_sort(timed.begin(), timed.end(),
+                     [](const auto& x, const auto& y) { return x.first < y.first; });
+    std::vector<lbm_batch_tuning> finalists{start};
+    for (const auto& tc : timed) {
+        if (finalists.size() > kFinalists)
+            break;
+        bool seen = false;
+        for (const lbm_batch_tuning& f : finalists)
+            seen = seen || same_tuning(f, tc.second);
+        if (!seen)
+            finalists.push_back(tc.second);
+    }
+    if (finalists.size() > 1) {
         const int long_steps = 4 * trial_steps;
-        float t_best = 0.f, t_start = 0.f;
-        for (int rep = 0; rep < 2; rep++) {
-            float t = 0.f;
-            if ((rc = step_time(best, long_steps, &t)))
-                return rc;
-            t_best += t;
-            if ((rc = step_time(start, long_steps, &t)))
-                return rc;
-            t_start += t;
-        }
-        if (t_start <= t_best)
-            best = start;
+        std::vector<float> total(finalists.size(), 0.f);
+        for (int rep = 0; rep < 2; rep++)
+            for (size_t i = 0; i < finalists.size(); i++) {
+                float t = 0.f;
+                if ((rc = step_time(finalists[i], long_steps, &t)))
+                    return rc;
+                total[i] += t;
+            }
+        size_t win = 0;
+        for (size_t i = 1; i < finalists.size(); i++)
+            if (total[i] < total[win])
+                win = i;
+        best = finalists[win];
     }
     if ((rc = apply_tuning(b, best)))
         return rc;

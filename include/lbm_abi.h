@@ -384,7 +384,7 @@ int lbm_residual(lbm_solver* solver, int layout, double* residual, hipStream_t s
 typedef struct lbm_batch lbm_batch;
 
 typedef struct lbm_batch_tuning {
-    int block;             /* threads per workgroup: 256 or 512 */
+    int block;             /* threads per workgroup: 256, 512 or 1024 (not FP64) */
     int cells_per_thread;  /* 1, 2 or 4 (tile = block * cells_per_thread cells) */
     int waves_per_cu;      /* 0: one workgroup per tile; k > 0: persistent grid of
                               k workgroups per CU striding over the tiles */
@@ -432,8 +432,9 @@ int lbm_batch_launch(lbm_batch* batch, int first_phase, int half_steps, hipStrea
  * options, member groups with the shapes again under the chosen group, and
  * the pair pass when the members allow it) as runs of trial_steps whole steps
  * (lbm_batch_run: graph-captured on a non-NULL stream, direct launches on the
- * NULL stream) and keeps the fastest; the winner must also beat the starting
- * tuning over longer runs.
+ * NULL stream); the four fastest distinct candidates and the starting tuning
+ * then run again over runs four times as long, interleaved, and the least
+ * total time wins (the starting tuning on a tie).
  * The trials are real steps: the members advance; reset them afterwards
  * (lbm_reset) when their state matters. Synchronous. */
 int lbm_batch_autotune(lbm_batch* batch, int trial_steps, hipStream_t stream,

@@ -35,7 +35,10 @@ TUNINGS = [dict(block=256, cells_per_thread=1, waves_per_cu=0, options=(1, 2)),
            dict(block=256, cells_per_thread=1, waves_per_cu=0, options=(0, 0), pairs=1),
            dict(block=256, cells_per_thread=1, waves_per_cu=0, options=(2, 0), pairs=2),
            dict(block=256, cells_per_thread=1, waves_per_cu=0, options=(1, 0), pairs=3),
-           dict(block=256, cells_per_thread=1, waves_per_cu=0, options=(3, 0), pairs=4)]
+           dict(block=256, cells_per_thread=1, waves_per_cu=0, options=(3, 0), pairs=4),
+           dict(block=1024, cells_per_thread=1, waves_per_cu=0, options=(1, 0)),
+           dict(block=1024, cells_per_thread=2, waves_per_cu=2, options=(9, 16)),
+           dict(block=1024, cells_per_thread=4, waves_per_cu=1, options=(16, 8), group_mb=1)]
 
 
 @pytest.mark.parametrize("precision", ["f32", "f64", "f16s", "bf16s", "f16c"])
@@ -46,6 +49,8 @@ def test_batch_members_equal_solo_runs(hip, precision, tuning):
     steps = 40
     if precision == "f64" and TUNINGS[tuning].get("pairs"):
         pytest.skip("the pair pass has no FP64 variant (covered by the rejection test)")
+    if precision == "f64" and TUNINGS[tuning]["block"] == 1024:
+        pytest.skip("FP64 batches take 256 or 512 threads (covered by the rejection test)")
     sp = specs(hip, precision)
     b = CavityBatch(sp, precision)
     b.set_tuning(**TUNINGS[tuning])
@@ -101,7 +106,7 @@ def test_autotune_then_reset_restores_initial_state(hip, on_stream):
     fresh = CavityBatch(sp, "f32")
     b = CavityBatch(sp, "f32")
     t = b.autotune(trial_steps=2, stream=st)
-    assert t["block"] in (256, 512) and t["cells_per_thread"] in (1, 2, 4)
+    assert t["block"] in (256, 512, 1024) and t["cells_per_thread"] in (1, 2, 4)
     for i in range(len(sp)):
         assert b.moments(i).tobytes() == fresh.moments(i).tobytes()
     b.run(40, stream=st)
@@ -174,3 +179,31 @@ def test_pair_pass_rejections(hip):
     b2.close()
     lib().lbm_batch_destroy(h)
     s.close()
+
+
+def test_workgroup_shape_rejections(hip):
+    """1024-thread workgroups are refused for FP64 batches (the kernels would
+    spill at 128 VGPRs per thread) and any other block size is an argument
+    error; a refused tuning leaves the batch stepping as before."""
+    from cavityflow_cuda_amd import LbmError, _lib
+    from cavityflow_cuda_amd.batch import CavityBatch
+    from cavityflow_cuda_amd._lib import lib
+    import ctypes
+    print("[case] batch workgroup shape rejections", flush=True)
+    sp = specs(hip, "f64")
+    b = CavityBatch(sp, "f64")
+    b.set_tuning(block=512, cells_per_thread=2)
+    before = b.tuning()
+    t = _lib.lbm_batch_tuning(1024, 1, 0, (ctypes.c_int * 2)(0, 0), 0, 0)
+    assert lib().lbm_batch_set_tuning(b._h, ctypes.byref(t)) == _lib.LBM_ERR_UNSUPPORTED
+    for blk in (128, 768, 2048):
+        t.block = blk
+        assert lib().lbm_batch_set_tuning(b._h, ctypes.byref(t)) == _lib.LBM_ERR_ARG
+    assert b.tuning() == before
+    with pytest.raises(LbmError):
+        b.set_tuning(block=1024)
+    b.run(8, chunk=4)
+    b.sync()
+    want = alone(hip, sp[0], "f64", 8)
+    assert b.moments(0).tobytes() == want.tobytes()
+    b.close()
