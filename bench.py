@@ -612,9 +612,11 @@ def ceiling_workload(args, cf, L, hip, stream):
     """Bandwidth ceilings of the step kernels' access patterns at the bench
     size and storage precision (`--precision`, FP32 by default): the library's
     pattern probes (lbm_probe_pattern: alpha's aligned 19-plane
-    read-modify-write, beta's shifted one, 19-plane read and write streams)
-    under the launch options the steps use and under workgroup shapes of
-    256-1024 threads and 1-8 cells per thread, a device-to-device copy, and
+    read-modify-write, beta's shifted one and its variants -- padded rows,
+    x or y / z shifts only, and for FP32 every access line-aligned with the x
+    shifts across lanes -- 19-plane read and write streams) under the launch
+    options the steps use and under workgroup shapes of 256-1024 threads and
+    1-8 cells per thread, a device-to-device copy, and
     the step kernels themselves (no exchange) for comparison. Each figure is
     bytes moved / mean launch time over `reps` launches timed with HIP events
     on the launch stream."""
@@ -638,7 +640,8 @@ def ceiling_workload(args, cf, L, hip, stream):
         return hip.elapsed_ms(e0, e1) / reps
 
     names = {0: "rmw_aligned", 1: "rmw_shifted", 2: "read", 3: "write",
-             4: "rmw_shifted_pitched", 5: "rmw_shifted_x", 6: "rmw_shifted_yz"}
+             4: "rmw_shifted_pitched", 5: "rmw_shifted_x", 6: "rmw_shifted_yz",
+             7: "rmw_lane_shifted"}
     vec = [256, 512, 256 | 16, 512 | 16, 256 | 8, 512 | 8, 256 | 4, 512 | 4, 512 | 1 | 16]
     # workgroup shapes: 256 / 512 / 1024 threads x 1 / 2 / 4 / 8 cells per
     # thread, uncapped or capped at 4 or 2 waves per SIMD (where one
@@ -652,10 +655,26 @@ def ceiling_workload(args, cf, L, hip, stream):
                 shapes += [b | sp | cap, b | sp | cap | 1]
     base = [0, 1, 16, 17, 8, 9, 4] + (vec if prec == "f32" else [])
     rmw = list(dict.fromkeys(base + shapes))
+    # the lane-shifted pattern (FP32): 1-8 consecutive 64-cell chunks per wave
+    lanes = [o for o in shapes if not (o & 2048 and o & 16384)]
     opt_sets = {0: rmw, 1: rmw, 2: [0, 1, 16], 3: [0, 1, 16], 4: [0, 16, 8, 4],
-                5: [0, 16, 8, 4], 6: [0, 16, 8, 4]}
+                5: [0, 16, 8, 4], 6: [0, 16, 8, 4], 7: lanes}
+    modes = (2, 3, 0, 1, 4, 5, 6) + ((7,) if prec == "f32" else ())
+    if args.patterns:
+        want = set(args.patterns.split(",")) | {"rmw_aligned", "rmw_shifted"}
+        unknown = want - set(names.values())
+        if unknown:
+            raise SystemExit(f"unknown patterns: {sorted(unknown)}")
+        modes = tuple(m for m in modes if names[m] in want)
+    if args.probe_options:
+        chosen = [int(v) for v in args.probe_options.split(",")]
+        # each pattern keeps the chosen values that its own sweep contains
+        opt_sets = {m: [o for o in chosen if o in set(opt_sets[m])] for m in modes}
+        for m in (0, 1):
+            if not opt_sets[m]:
+                raise SystemExit(f"--probe-options: none accepted by {names[m]}")
     probes = []
-    for mode in (2, 3, 0, 1, 4, 5, 6):
+    for mode in modes:
         for opt in opt_sets[mode]:
             cells = ctypes.c_uint64()
             check(L.lbm_probe_pattern(s.handle, mode, opt, ctypes.byref(cells), stream))
@@ -807,6 +826,12 @@ def main():
                     choices=("step", "precision", "batched", "serving", "ceiling", "split"))
     ap.add_argument("--split", type=int, nargs=3, default=(2, 2, 2),
                     help="split: subdomains per axis")
+    ap.add_argument("--patterns", default=None,
+                    help="ceiling: comma-separated probe patterns to run (rmw_aligned and "
+                         "rmw_shifted always run; default all)")
+    ap.add_argument("--probe-options", default=None,
+                    help="ceiling: comma-separated lbm_probe_pattern option values to run "
+                         "instead of the sweeps (each pattern takes the ones it accepts)")
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--cavities", type=int, default=64, help="batched: number of cavities")
     ap.add_argument("--no-graph", action="store_true",

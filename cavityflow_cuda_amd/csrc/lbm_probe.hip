@@ -14,6 +14,11 @@
 //                 shifts line-aligned for FP32, x shifts by one element).
 //   RMW_SHIFTED_X: the x components of the shifts only (one element).
 //   RMW_SHIFTED_YZ: the y and z components only (nx and nx ny elements).
+//   RMW_LANE_SHIFTED (FP32): the padded-row shifts with the x component
+//                 taken across lanes: every wave loads and stores whole
+//                 aligned 64-cell chunks of a plane, moves the values one
+//                 lane up or down (DPP wave shifts), and adds one element
+//                 beyond either end of its run of consecutive chunks.
 //   READ        : 19 loads per cell, nothing stored.
 //   WRITE       : 19 stores per cell.
 // Launch options: LBM_LAUNCH_NONTEMPORAL and the WAVES4/3/2 occupancy caps,
@@ -164,6 +169,118 @@ __global__ __launch_bounds__(BLOCK) void k_probe(ProbeArgs a)
     }
 }
 
+// RMW_LANE_SHIFTED: wave w of a workgroup takes R consecutive 64-cell chunks
+// from a line-aligned start; plane l's base carries the (line-aligned) y / z
+// shift. For c_x = +1 cell i takes slot i + 1: lane j gets lane j + 1 of its
+// chunk (wave_shl:1), lane 63 lane 0 of the next chunk, the last chunk's lane
+// 63 the element past the run, loaded on its own; c_x = -1 mirrors it. The
+// stores move the values back: each chunk's aligned slots, less the one slot
+// a neighbouring wave owns, plus that end element as a single-lane store.
+__device__ inline float lane_from(float old, float v, int ctrl)
+{
+    // ctrl is 0x130 (wave_shl:1, lane j reads lane j + 1) or 0x138 (wave_shr:1,
+    // lane j reads lane j - 1); the lane with no source keeps old
+    return __int_as_float(ctrl == 0x130
+        ? __builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), 0x130, 0xf, 0xf, false)
+        : __builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+
+__device__ inline float lane_value(float v, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+template <bool NT>
+__device__ inline float ld(const char* base, uint32_t cell)
+{
+    const __attribute__((address_space(1))) float* p =
+        (const __attribute__((address_space(1))) float*)(base + cell * (uint32_t)sizeof(float));
+    return NT ? __builtin_nontemporal_load(p) : *p;
+}
+
+template <bool NT>
+__device__ inline void st(const char* base, uint32_t cell, float v)
+{
+    uint32_t o = cell * (uint32_t)sizeof(float);
+    asm volatile("" : "+v"(o));
+    __attribute__((address_space(1))) float* p =
+        (__attribute__((address_space(1))) float*)(const_cast<char*>(base) + o);
+    if (NT)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
+}
+
+template <bool NT, int BLOCK, int R>
+__global__ __launch_bounds__(BLOCK) void k_probe_lanes(ProbeArgs a)
+{
+    constexpr int kShl = 0x130, kShr = 0x138;
+    const int lane = (int)(threadIdx.x & 63);
+    const uint32_t w0 =
+        a.first + (blockIdx.x * (uint32_t)(BLOCK / 64) + (threadIdx.x >> 6)) * (uint32_t)(64 * R);
+    if (w0 >= a.last)
+        return;   // [first, last) is whole chunks: every wave below has 1..R of them
+    const int nch = (int)min((a.last - w0) / 64u, (uint32_t)R);
+    float v[kQ][R];
+#pragma unroll
+    for (int l = 0; l < kQ; l++) {
+#pragma unroll
+        for (int k = 0; k < R; k++)
+            v[l][k] = k < nch ? ld<NT>(a.plane[l], w0 + 64 * k + lane) : 0.f;
+    }
+    float e[kQ];
+#pragma unroll
+    for (int l = 0; l < kQ; l++)
+        e[l] = cx(l) > 0 ? ld<NT>(a.plane[l], w0 + 64 * nch)
+               : cx(l) < 0 ? ld<NT>(a.plane[l], w0 - 1) : 0.f;
+    // the cells' values, as a step would take them
+#pragma unroll
+    for (int l = 0; l < kQ; l++) {
+        float c[R];
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            if (k >= nch)
+                break;
+            if (cx(l) > 0)
+                c[k] = lane_from(k + 1 < R && k + 1 < nch ? lane_value(v[l][k + 1 < R ? k + 1 : k], 0) : e[l],
+                                 v[l][k], kShl);
+            else if (cx(l) < 0)
+                c[k] = lane_from(k > 0 ? lane_value(v[l][k > 0 ? k - 1 : 0], 63) : e[l], v[l][k], kShr);
+            else
+                c[k] = v[l][k];
+            c[k] = keep(c[k], a);
+        }
+        // back to the slots they came from. Every cross-lane read comes
+        // before the lane-dependent stores: inside a branch that leaves lane 63
+        // inactive the compiler may reuse that lane's registers, and a
+        // readlane there returns whatever they then hold.
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            if (k >= nch)
+                break;
+            const uint32_t cell = w0 + 64 * k + lane;
+            if (cx(l) > 0) {
+                const float w = lane_from(k > 0 ? lane_value(c[k > 0 ? k - 1 : 0], 63) : 0.f, c[k], kShr);
+                const float end = lane_value(c[k], 63);
+                if (k > 0 || lane > 0)
+                    st<NT>(a.plane[l], cell, w);
+                if (k == nch - 1 && lane == 0)
+                    st<NT>(a.plane[l], w0 + 64 * nch, end);
+            } else if (cx(l) < 0) {
+                const float w = lane_from(k + 1 < R && k + 1 < nch ? lane_value(c[k + 1 < R ? k + 1 : k], 0) : 0.f,
+                                          c[k], kShl);
+                const float end = lane_value(c[k], 0);
+                if (k < nch - 1 || lane < 63)
+                    st<NT>(a.plane[l], cell, w);
+                if (k == 0 && lane == 0)
+                    st<NT>(a.plane[l], w0 - 1, end);
+            } else {
+                st<NT>(a.plane[l], cell, c[k]);
+            }
+        }
+    }
+}
+
 // Dynamic LDS that caps a CU at m resident workgroups (160 KB of LDS per CU).
 size_t cap_lds(int m)
 {
@@ -180,6 +297,33 @@ hipError_t launch_vec(const ProbeArgs& a, int opts, size_t lds, hipStream_t st)
     else
         hipLaunchKernelGGL((k_probe_vec<MODE, false, VEC>), dim3(blocks), dim3(256), lds, st, a);
     return hipGetLastError();
+}
+
+template <int BLOCK, int R>
+hipError_t launch_lanes_r(const ProbeArgs& a, int opts, size_t lds, hipStream_t st)
+{
+    const uint32_t chunks = (a.last - a.first) / 64;
+    const uint32_t per = (uint32_t)(BLOCK / 64 * R);
+    const uint32_t blocks = (chunks + per - 1) / per;
+    if (opts & LBM_LAUNCH_NONTEMPORAL)
+        hipLaunchKernelGGL((k_probe_lanes<true, BLOCK, R>), dim3(blocks), dim3(BLOCK), lds, st, a);
+    else
+        hipLaunchKernelGGL((k_probe_lanes<false, BLOCK, R>), dim3(blocks), dim3(BLOCK), lds, st,
+                           a);
+    return hipGetLastError();
+}
+
+template <int BLOCK>
+hipError_t launch_lanes(const ProbeArgs& a, int opts, size_t lds, hipStream_t st)
+{
+    if constexpr (BLOCK < 1024)
+        if (opts & LBM_PROBE_SPAN8)
+            return launch_lanes_r<BLOCK, 8>(a, opts, lds, st);
+    if (opts & LBM_PROBE_SPAN4)
+        return launch_lanes_r<BLOCK, 4>(a, opts, lds, st);
+    if (opts & LBM_PROBE_SPAN2)
+        return launch_lanes_r<BLOCK, 2>(a, opts, lds, st);
+    return launch_lanes_r<BLOCK, 1>(a, opts, lds, st);
 }
 
 template <class E, int MODE, int BLOCK, int SPAN>
@@ -218,6 +362,10 @@ hipError_t launch_mode(const ProbeArgs& a, int opts, hipStream_t st)
     // a cap of w waves per SIMD leaves w * 256 / block workgroups per CU
     const size_t lds = waves ? cap_lds(waves * 256 / block) : 0;
     if constexpr (sizeof(E) == 4) {
+        if (MODE == LBM_PROBE_RMW_LANE_SHIFTED)
+            return block == 1024 ? launch_lanes<1024>(a, opts, lds, st)
+                   : block == 512 ? launch_lanes<512>(a, opts, lds, st)
+                                  : launch_lanes<256>(a, opts, lds, st);
         if (MODE != LBM_PROBE_READ && MODE != LBM_PROBE_WRITE) {
             if (opts & LBM_PROBE_VEC4)
                 return launch_vec<MODE, 4>(a, opts, lds, st);
@@ -239,6 +387,8 @@ hipError_t launch_probe(const ProbeArgs& a, int mode, int opts, hipStream_t st)
     case LBM_PROBE_RMW_ALIGNED: return launch_mode<E, LBM_PROBE_RMW_ALIGNED>(a, opts, st);
     case LBM_PROBE_READ: return launch_mode<E, LBM_PROBE_READ>(a, opts, st);
     case LBM_PROBE_WRITE: return launch_mode<E, LBM_PROBE_WRITE>(a, opts, st);
+    case LBM_PROBE_RMW_LANE_SHIFTED:
+        return launch_mode<E, LBM_PROBE_RMW_LANE_SHIFTED>(a, opts, st);
     default:   // the shifted variants differ only in their plane bases
         return launch_mode<E, LBM_PROBE_RMW_SHIFTED>(a, opts, st);
     }
@@ -256,7 +406,7 @@ extern "C" int lbm_probe_pattern(lbm_solver* s, int mode, int options, uint64_t*
     const int known = LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_WAVES4 | LBM_LAUNCH_WAVES3 |
                       LBM_LAUNCH_WAVES2 | LBM_PROBE_VEC2 | LBM_PROBE_VEC4 | LBM_PROBE_BLOCK512 |
                       LBM_PROBE_BLOCK1024 | LBM_PROBE_SPAN2 | LBM_PROBE_SPAN4 | LBM_PROBE_SPAN8;
-    if (!s || mode < LBM_PROBE_RMW_ALIGNED || mode > LBM_PROBE_RMW_SHIFTED_YZ ||
+    if (!s || mode < LBM_PROBE_RMW_ALIGNED || mode > LBM_PROBE_RMW_LANE_SHIFTED ||
         (options & ~known)) {
         set_error("lbm_probe_pattern: invalid argument");
         return LBM_ERR_ARG;
@@ -285,8 +435,15 @@ extern "C" int lbm_probe_pattern(lbm_solver* s, int mode, int options, uint64_t*
         return LBM_ERR_ARG;
     }
     const size_t elem = (size_t)s->elem_bytes;
-    if (vec && s->p.precision != LBM_F32) {
-        set_error("lbm_probe_pattern: vector accesses on FP32 solvers only");
+    const bool lanes = mode == LBM_PROBE_RMW_LANE_SHIFTED;
+    if (lanes && (vec || ((options & LBM_PROBE_SPAN8) && block == 1024))) {
+        set_error("lbm_probe_pattern: the lane-shifted pattern takes no vector accesses, "
+                  "and 1024-thread workgroups at most 4 chunks per wave");
+        return LBM_ERR_ARG;
+    }
+    if ((vec || lanes) && s->p.precision != LBM_F32) {
+        set_error("lbm_probe_pattern: vector accesses and the lane-shifted pattern on FP32 "
+                  "solvers only");
         return LBM_ERR_UNSUPPORTED;
     }
     if ((uint64_t)s->stride * elem >= (1ull << 32)) {
@@ -306,17 +463,27 @@ extern "C" int lbm_probe_pattern(lbm_solver* s, int mode, int options, uint64_t*
         case LBM_PROBE_RMW_SHIFTED_PITCHED: shift[l] = cx(l) + cy(l) * pitch + cz(l) * pxy; break;
         case LBM_PROBE_RMW_SHIFTED_X: shift[l] = cx(l); break;
         case LBM_PROBE_RMW_SHIFTED_YZ: shift[l] = cy(l) * nx + cz(l) * nxy; break;
+        // the x shift is taken across lanes; one more element at either end
+        case LBM_PROBE_RMW_LANE_SHIFTED: shift[l] = cy(l) * pitch + cz(l) * pxy; break;
         default: shift[l] = 0;
         }
         span = std::max<int64_t>(span, shift[l] < 0 ? -shift[l] : shift[l]);
     }
-    if (2 * span >= s->n) {
+    if (lanes)
+        span += 1;
+    // every accessed element id + shift inside [0, n); the lane-shifted
+    // pattern covers whole 64-cell chunks from a 64-cell boundary
+    int64_t first = span, last = s->n - span;
+    if (lanes) {
+        first = (first + 63) / 64 * 64;
+        last = last < first ? first : first + (last - first) / 64 * 64;
+    }
+    if (2 * span >= s->n || last <= first) {
         set_error("lbm_probe_pattern: domain too small for this pattern");
         return LBM_ERR_ARG;
     }
-    // every accessed element id + shift inside [0, n)
-    a.first = (uint32_t)span;
-    a.last = (uint32_t)(s->n - span);
+    a.first = (uint32_t)first;
+    a.last = (uint32_t)last;
     a.zero = 0.0f;
     a.zero_bits = 0;
     a.never = 0xFFFFFFFFu;

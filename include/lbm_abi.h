@@ -483,7 +483,13 @@ enum lbm_probe_mode {
                                    shifts c_x + c_y P + c_z P ny_h, P = nx_h rounded
                                    up to 32 */
     LBM_PROBE_RMW_SHIFTED_X = 5,  /* the x components of beta's shifts only */
-    LBM_PROBE_RMW_SHIFTED_YZ = 6  /* the y and z components only */
+    LBM_PROBE_RMW_SHIFTED_YZ = 6, /* the y and z components only */
+    LBM_PROBE_RMW_LANE_SHIFTED = 7 /* FP32: beta's pattern over rows padded to 128 bytes
+                                   with the x shifts taken across lanes: each wave
+                                   loads and stores whole aligned 64-cell chunks (its
+                                   LBM_PROBE_SPAN2 / 4 / 8 = that many consecutive
+                                   chunks, default 1; SPAN8 not with BLOCK1024) plus
+                                   one element past either end of its run */
 };
 /* options: LBM_LAUNCH_NONTEMPORAL and LBM_LAUNCH_WAVES4/3/2 (a cap must hold
  * one workgroup: WAVES3 / WAVES2 exclude BLOCK1024), and for the

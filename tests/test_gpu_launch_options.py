@@ -12,14 +12,14 @@ pytestmark = pytest.mark.gpu
 SIZE = (37, 21, 13)            # odd sizes: partial workgroups and wrap blocks
 
 
-def state(hip, precision, rng):
+def state(hip, precision, rng, size=SIZE):
     from cavityflow_cuda_amd import _lib
-    s = hip.LbmSolverHIP(SIZE, (_lib.OBSTACLE, _lib.GHOST_LAYER, _lib.OBSTACLE,
+    s = hip.LbmSolverHIP(size, (_lib.OBSTACLE, _lib.GHOST_LAYER, _lib.OBSTACLE,
                                 _lib.VELOCITY_INJECTION, _lib.GHOST_LAYER, _lib.OBSTACLE),
                          precision, inv_tau=1.37, lid_velocity=0.04,
                          gravitation=(1e-4, -2e-4, 5e-5), store_densities=True,
                          store_velocities=True)
-    nh = tuple(v + 2 for v in SIZE)
+    nh = tuple(v + 2 for v in size)
     full = ((0, 0, 0), nh)
     cells = int(np.prod(nh))
     flags = rng.choice(np.array([1, 2, 4, 8], np.int32), cells, p=[0.15, 0.7, 0.1, 0.05])
@@ -184,11 +184,40 @@ def test_pattern_probes_keep_or_reset_the_state(hip, prec):
     w = np.array([1 / 18] * 4 + [1 / 36] * 12 + [1 / 18] * 2 + [1 / 3], dt)
     got = s.getDensityDistributions(*full).reshape(19, -1)
     assert np.array_equal(got, np.repeat(w[:, None], got.shape[1], axis=1))
-    bad = [(7, 0), (0, 2), (0, 32768), (0, 256 | 4096), (0, 1024 | 2048), (0, 4096 | 8192),
-           (2, 4096), (3, 1024), (0, 2048 | 16), (0, 2048 | 8)]
+    bad = [(8, 0), (-1, 0), (0, 2), (0, 32768), (0, 256 | 4096), (0, 1024 | 2048),
+           (0, 4096 | 8192), (2, 4096), (3, 1024), (0, 2048 | 16), (0, 2048 | 8),
+           (7, 256), (7, 512 | 1), (7, 16384 | 2048), (7, 2048 | 16)]
     if prec != "f32":
-        bad.append((0, 256))
+        bad += [(0, 256), (7, 0)]
     for mode, opt in bad:
         with pytest.raises(LbmError):
             check(lib().lbm_probe_pattern(s.handle, mode, opt, None, None))
     s.close()
+
+
+def test_lane_shifted_probe_keeps_the_state(hip):
+    """The lane-shifted FP32 probe (padded-row y / z shifts, x shifts moved
+    across lanes with DPP wave shifts, one extra element at either end of a
+    wave's run of chunks) stores every population back where it was loaded,
+    for runs of 1-8 chunks per wave, including the short runs of the last
+    workgroup (175 chunks here: 7 past a multiple of 8, 15 past 16 and 32) --
+    a shift in the wrong direction or a lost end element changes the array."""
+    import ctypes
+    from cavityflow_cuda_amd._lib import check, lib
+    print("[case] lane-shifted probe", flush=True)
+    for size in ((37, 23, 13), SIZE):
+        s, full = state(hip, "f32", np.random.default_rng(5), size)
+        before = s.getDensityDistributions(*full).copy()
+        nx, ny, nz = s.size_h
+        pitch = (nx + 31) // 32 * 32
+        span = pitch * ny + pitch + 1
+        first = (span + 63) // 64 * 64
+        cells = (nx * ny * nz - span - first) // 64 * 64
+        for opt in (0, 1, 16, 4, 1024, 2048 | 4, 4096, 8192 | 1024 | 1, 16384 | 17, 16384 | 1024,
+                    8192 | 2048, 4096 | 2048 | 1, 8192, 16384):
+            n = ctypes.c_uint64()
+            check(lib().lbm_probe_pattern(s.handle, 7, opt, ctypes.byref(n), None))
+            s.sync()
+            assert n.value == cells, (size, opt)
+            assert s.getDensityDistributions(*full).tobytes() == before.tobytes(), (size, opt)
+        s.close()
