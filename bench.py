@@ -5,8 +5,12 @@ Default workload: the lid-driven cavity at one rank's real subdomain of a
 10.3 GB of FP32 populations), Re 1000, lid velocity 0.1, FP32 -- with all six
 face halos packed (CComm face shapes, 514 x 514 cells; the 5 populations per face
 and phase that the receiving interior reads) and unpacked on the device after
-every half-step. The unpack targets a shadow subdomain of the same shape so the
-cavity walls stay physical.
+every half-step. The y / z unpacks target a shadow subdomain of the same shape so
+the cavity walls stay physical; for FP32 the x faces are packed and unpacked inside
+the step kernels (lbm_set_step_pack / lbm_set_step_unpack), each face receiving its
+own pack of the step pair before -- a wall reflected through the exchange path, which
+leaves the cavity unchanged -- from one of two buffers while packing into the
+other (for the 16-bit formats the x faces use the copy kernels too).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 512]
                     [--precision f32|f64|f16s|bf16s|f16c] [--no-halo]
@@ -479,6 +483,31 @@ def step_run(args, cf, L, hip, stream, precision, steps, warmup):
     if fused:
         # the step kernels pack the LEFT/RIGHT faces themselves
         check(L.lbm_set_step_pack(solver.handle, bufs[0], bufs[1]))
+    # x faces exchanged inside the steps: each x face takes its received
+    # populations from its own fused-pack buffer -- a wall reflected through
+    # the exchange path (the pack stores what the wall's bounce-back returns,
+    # so the cavity is unchanged), with the bytes and access pattern of a
+    # neighbour's exchange -- and no x-face copy kernel runs
+    # measured at 512^3: fused 33.8 k vs 33.6 k MLUPS FP32; the 16-bit betas
+    # lose 14-22 % with the received loads (F16S 57.2 k vs 60.3 k)
+    x_mode = args.x_exchange if args.x_exchange != "auto" else (
+        "fused" if precision == "f32" else "kernels")
+    x_in_steps = fused and x_mode == "fused"
+    xbufs = []
+    if x_in_steps:
+        # two buffers per x face: each pair of steps packs into one and
+        # unpacks from the other, as with a neighbour, whose steps write the
+        # other phase's planes (other cache lines) of the buffer one reads
+        elem = 8 if precision == "f64" else 4
+        spare = []
+        for d in (0, 1):
+            p = ctypes.c_void_p()
+            check(L.lbm_device_alloc(device, solver.face_elements(d) * elem, ctypes.byref(p)))
+            spare.append(p)
+        xbufs = [bufs[:2], spare]
+        for d in (0, 1):
+            for phase in (0, 1):
+                check(L.lbm_pack_face(solver.handle, d, phase, xbufs[1][d], stream))
     # as launched (defaults may depend on the step pack)
     launch_opts = []
     for k in (0, 1):
@@ -521,6 +550,8 @@ def step_run(args, cf, L, hip, stream, precision, steps, warmup):
             return host_halo_exchange(phase)
         # reference order: x faces, then y, then z (CController.cpp:369-570)
         for axis in range(3):
+            if x_in_steps and axis == 0:
+                continue
             pair = yz_pairs if axis else not separate_unpack
             if not (fused and axis == 0):
                 if axis and yz_pairs:
@@ -543,6 +574,10 @@ def step_run(args, cf, L, hip, stream, precision, steps, warmup):
     def one_step(events=None):
         n = counter[0]
         phase = n & 1
+        if x_in_steps:
+            j = (n >> 1) & 1
+            check(L.lbm_set_step_pack(solver.handle, xbufs[j][0], xbufs[j][1]))
+            check(L.lbm_set_step_unpack(solver.handle, xbufs[1 - j][0], xbufs[1 - j][1]))
         if events is not None:
             hip.record(events[0], stream)
         check(step_fns[phase](solver.handle, None, None, stream))
@@ -588,13 +623,14 @@ def step_run(args, cf, L, hip, stream, precision, steps, warmup):
     c.close()
     if shadow is not None:
         shadow.close()
-    for p in bufs:
+    for p in bufs + (xbufs[1] if xbufs else []):
         L.lbm_device_free(device, p)
     for p in hbufs:
         hip.host_free(p)
 
     return dict(size=size, params=params, mlups=mlups, total_ms=total_ms, kern=kern, dom=dom,
                 achieved=achieved, traffic=traffic, alg_bytes=alg_bytes, halo=halo, fused=fused,
+                x_in_steps=x_in_steps,
                 host_ex=host_ex, separate_unpack=separate_unpack, launch_opts=launch_opts,
                 wall=wall, halo_ms=halo_ms, step_ms_total=step_ms_total, steps=steps)
 
@@ -808,6 +844,11 @@ def main():
                     help="step: faces pass through pinned host buffers with the reference's "
                          "calls (getDensityDistributions / setDensityDistributions), "
                          "the PCIe-inclusive rate")
+    ap.add_argument("--x-exchange", choices=("auto", "fused", "kernels"), default="auto",
+                    help="x faces: packed and unpacked inside the step kernels (fused; each "
+                         "face receives its own pack, a reflected wall) or unpacked by "
+                         "lbm_unpack_face into the shadow subdomain (kernels); auto: fused "
+                         "for FP32, kernels for the other formats (measured at 512^3)")
     ap.add_argument("--no-fused-pack", action="store_true",
                     help="pack the x faces with separate kernels instead of inside the steps")
     ap.add_argument("--cpu-sample-steps", type=int, default=2)
@@ -934,7 +975,11 @@ def main():
                          f"({size[0] + 2}^3 with walls), {args.precision} populations, "
                          f"Re {args.reynolds:g}, lid {args.u_lid:g}"
                          + ((", six-face exchange through pinned host buffers every half-step"
-                             if host_ex else ", six-face pack+unpack on the device every half-step")
+                             if host_ex else
+                             (", six-face pack+unpack on the device every half-step (x faces "
+                              "inside the step kernels, y / z faces by copy kernels)"
+                              if run["x_in_steps"] else
+                              ", six-face pack+unpack on the device every half-step"))
                             if halo else "")),
             "size": list(size),
             "reynolds": args.reynolds,
@@ -942,7 +987,8 @@ def main():
             "tau": round(params["tau"], 6),
             "precision": args.precision,
             "halo_exchange": halo,
-            "fused_x_pack": fused, "paired_unpack": halo and not separate_unpack and not host_ex,
+            "fused_x_pack": fused, "fused_x_unpack": run["x_in_steps"],
+            "paired_unpack": halo and not separate_unpack and not host_ex and not run["x_in_steps"],
             "paired_yz_copies": halo and not host_ex and args.yz_copies == "pair",
             "host_exchange": host_ex,
             "parallelism": "single GPU (one rank's subdomain)",

@@ -34,7 +34,8 @@ EXPORTED = (
     "lbm_face_box", "lbm_face_elements", "lbm_pack_face", "lbm_unpack_face",
     "lbm_unpack_face_pair", "lbm_pack_face_pair",
     "lbm_exchange_face", "lbm_device_alloc", "lbm_device_free",
-    "lbm_set_launch_options", "lbm_get_launch_options", "lbm_set_step_pack", "lbm_get_moments",
+    "lbm_set_launch_options", "lbm_get_launch_options", "lbm_set_step_pack",
+    "lbm_set_step_unpack", "lbm_get_moments",
     "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_size", "lbm_batch_set_tuning",
     "lbm_batch_get_tuning", "lbm_batch_step", "lbm_batch_run", "lbm_batch_launch",
     "lbm_batch_autotune",
@@ -163,6 +164,7 @@ def lib() -> ctypes.CDLL:
         "lbm_set_launch_options": ([P, I, I], I),
         "lbm_get_launch_options": ([P, I, ctypes.POINTER(I)], I),
         "lbm_set_step_pack": ([P, P, P], I),
+        "lbm_set_step_unpack": ([P, P, P], I),
         "lbm_get_moments": ([P, I, i3, i3, P, I, P], I),
         "lbm_batch_create": ([ctypes.POINTER(P), I, ctypes.POINTER(P)], I),
         "lbm_batch_destroy": ([P], I),

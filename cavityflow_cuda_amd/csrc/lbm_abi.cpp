@@ -838,6 +838,22 @@ int lbm_set_step_pack(lbm_solver* s, void* left_buf, void* right_buf)
     return LBM_OK;
 }
 
+int lbm_set_step_unpack(lbm_solver* s, const void* left_buf, const void* right_buf)
+{
+    if (!s) {
+        set_error("lbm_set_step_unpack: solver is NULL");
+        return LBM_ERR_ARG;
+    }
+    if ((left_buf || right_buf) && s->size_h[0] < 4) {
+        // the kernels take each send-plane cell as one face's
+        set_error("lbm_set_step_unpack: the x extent must be at least 2 interior cells");
+        return LBM_ERR_UNSUPPORTED;
+    }
+    s->xunpack[0] = left_buf;
+    s->xunpack[1] = right_buf;
+    return LBM_OK;
+}
+
 int lbm_exchange_face(lbm_solver* from, int direction, int phase, lbm_solver* to,
                       hipStream_t stream)
 {

@@ -60,6 +60,7 @@ __device__ __forceinline__ StepArgs<C> member_args(const MemberDesc& d, int phas
     a.store_u = d.store_u;
     a.xpack_on = 0;
     a.xpack[0] = a.xpack[1] = nullptr;
+    a.xunpack[0] = a.xunpack[1] = nullptr;
     a.status = d.status;
 #pragma unroll
     for (int l = 0; l < kQ; l++)
@@ -115,7 +116,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch(const MemberDesc* __restrict__ 
                 const uint32_t id = base + (uint32_t)k * BLOCK + threadIdx.x;
                 if (id < (uint32_t)a.n) {
                     const Acc32<S, NT> acc{a.plane, id * (uint32_t)sizeof(S)};
-                    alpha_cell<P>(a, id, ((const LBM_GLOBAL uint8_t*)a.flags)[id], acc);
+                    alpha_cell<P, kXfNone>(a, id, ((const LBM_GLOBAL uint8_t*)a.flags)[id], acc);
                 }
             }
         } else {
@@ -129,10 +130,10 @@ __global__ __launch_bounds__(BLOCK) void k_batch(const MemberDesc* __restrict__ 
                 if (wrap) {
                     // the wrap path addresses planes from the unshifted base
                     const Acc64<S, true> acc{(S*)plane0, a.stride, (int32_t)id, a.n};
-                    beta_cell<P>(a, id, ((const LBM_GLOBAL uint8_t*)a.flags)[id], acc);
+                    beta_cell<P, kXfNone>(a, id, ((const LBM_GLOBAL uint8_t*)a.flags)[id], acc);
                 } else {
                     const Acc32<S, false> acc{a.plane, id * (uint32_t)sizeof(S)};
-                    beta_cell<P>(a, id, ((const LBM_GLOBAL uint8_t*)a.flags)[id], acc);
+                    beta_cell<P, kXfNone>(a, id, ((const LBM_GLOBAL uint8_t*)a.flags)[id], acc);
                 }
             }
         }

@@ -98,13 +98,14 @@ struct Acc64 {
 
 // ------------------------------------------------------- fused x packing ---
 
-// Whether the cell lies in the LEFT (x = 1) / RIGHT (x = nx - 2) send plane;
-// row = z * ny + y is its index in the dense face buffers.
+// Whether the cell lies in the LEFT (x = 1) / RIGHT (x = nx - 2) send plane
+// with a fused pack (pl / pr) or unpack (ul / ur) buffer on that side;
+// row = z * ny + y is its index in the dense face buffers. Returns pl || pr.
 template <class C>
 __device__ __forceinline__ bool xpack_where(const StepArgs<C>& a, uint32_t id, uint32_t& row,
-                                            bool& pl, bool& pr)
+                                            bool& pl, bool& pr, bool& ul, bool& ur)
 {
-    pl = pr = false;
+    pl = pr = ul = ur = false;
     row = 0;
     if (!a.xpack_on)
         return false;
@@ -112,6 +113,8 @@ __device__ __forceinline__ bool xpack_where(const StepArgs<C>& a, uint32_t id, u
     const int x = (int)(id - row * (uint32_t)a.nx);
     pl = x == 1 && a.xpack[0] != nullptr;
     pr = x == a.nx - 2 && a.xpack[1] != nullptr;
+    ul = x == 1 && a.xunpack[0] != nullptr;
+    ur = x == a.nx - 2 && a.xunpack[1] != nullptr;
     return pl || pr;
 }
 
@@ -153,20 +156,82 @@ __device__ __forceinline__ void xpack_beta(const StepArgs<typename P::C>& a, uin
     }
 }
 
+// ------------------------------------------------------ fused x unpacking ---
+
+// The face work compiled into a step kernel: none, the fused x-face pack
+// (lbm_set_step_pack), or the pack and the fused x-face unpack
+// (lbm_set_step_unpack); a kernel without a part carries none of its code.
+constexpr int kXfNone = 0, kXfPack = 1, kXfUnpack = 2;
+
+// A received x-face buffer brings one plane of each opposite pair
+// (xplane(k), xplane(k) + 1) = (c_x = +1, c_x = -1), k = 0..4: before alpha
+// the send-plane cell's own slots of the vectors pointing into the solver
+// through its face (c_x = +1 on the LEFT send plane, -1 on the RIGHT one:
+// what lbm_unpack_face(phase 1) stores there); before beta the halo-plane
+// slots the cell reads (c_x = -1 in the LEFT halo plane, +1 in the RIGHT one:
+// what lbm_unpack_face(phase 0) stores there). `odd` selects the c_x = -1
+// plane of each pair. Opposite vectors share their lattice weight, so both
+// planes of a pair decode with xplane(k).
+__host__ __device__ constexpr int xplane(int k) { return k == 0 ? 0 : 2 * k + 2; }
+
+template <int STEP>
+__device__ __forceinline__ int xunpack_odd(bool ul)
+{
+    return STEP == 0 ? (ul ? 0 : 1) : (ul ? 1 : 0);
+}
+
+// Loads a face cell's 5 received values (ul: from the LEFT buffer, else the
+// RIGHT one), issued with its slot loads. A beta cell's halo slots sit at the
+// rows of their halo-plane cells (the reference linear wrap, as in
+// xpack_beta).
+template <class P, int STEP>
+__device__ __forceinline__ void xunpack_load(const StepArgs<typename P::C>& a, uint32_t row,
+                                             bool ul, int odd, typename P::C (&u)[5])
+{
+    const typename P::C* buf = a.xunpack[ul ? 0 : 1];
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+        const int e = xplane(k);
+        int32_t t = (int32_t)row;
+        if (STEP == 1) {
+            const int32_t off = cy(e) + cz(e) * a.ny;   // the c_x = -1 plane's is -off
+            t += odd ? -off : off;
+            t = t < 0 ? t + a.fcells : t;
+            t = t >= a.fcells ? t - a.fcells : t;
+        }
+        u[k] = buf[(int64_t)(e + odd) * a.fcells + t];
+    }
+}
+
+// The value a step would load from the slot after lbm_unpack_face: the
+// received one through the storage type, its range marked in ov as the
+// unpack marks it.
+template <class P>
+__device__ __forceinline__ typename P::C xunpack_value(typename P::C r, int l, uint32_t& ov)
+{
+    return P::load(P::store_checked(r, l, ov), l);
+}
+
 // ----------------------------------------------------------------- alpha ---
 
 // One cell of CLbmAlphaCPU::alphaKernelCPU (CLbmAlphaCPU.cpp:75-494). The
 // reference's `tmp` is a #define for `rho` (:199): every gravity term
 // overwrites rho and the stored density is the last gravity term. `flag` is
 // the cell's flag, passed in so a caller that already holds it skips a load.
-template <class P, class A>
+template <class P, int XF = kXfPack, class A>
 __device__ __forceinline__ void alpha_cell(const StepArgs<typename P::C>& a, uint32_t id,
                                            int flag, const A& acc)
 {
     using C = typename P::C;
-    uint32_t row;
-    bool pl, pr;
-    const bool pack = xpack_where(a, id, row, pl, pr);
+    uint32_t row = 0;
+    bool pl = false, pr = false, ul = false, ur = false;
+    if (XF != kXfNone)
+        xpack_where(a, id, row, pl, pr, ul, ur);
+    if (XF != kXfUnpack)
+        ul = ur = false;
+    const bool pack = pl || pr;
+    const bool unpack = ul || ur;
+    const int odd = xunpack_odd<0>(ul);
     // The slots are loaded for every cell, whatever its flag, so that the
     // flag's latency overlaps theirs instead of preceding them. GHOST_LAYER
     // (skipped by the reference, :75-76) and OBSTACLE without outputs then
@@ -175,9 +240,39 @@ __device__ __forceinline__ void alpha_cell(const StepArgs<typename P::C>& a, uin
 #pragma unroll
     for (int l = 0; l < kQ; l++)
         d[l] = P::load(acc.load(l, 0), l);
+    C u[5];
+    if (unpack)
+        xunpack_load<P, 0>(a, row, ul, odd, u);
     // keeps the flag test (and its wait) below the issued loads
     __builtin_amdgcn_sched_barrier(0);
+    if (unpack) {
+        uint32_t uov = 0;
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            const int e = xplane(k);
+            const C v = xunpack_value<P>(u[k], e, uov);
+            d[e] = odd ? d[e] : v;
+            d[e + 1] = odd ? v : d[e + 1];
+        }
+        raise_overflow<P>(a.status, uov);
+    }
+    // a cell that stores nothing leaves the received values where the
+    // unpack would have put them: the unpack's codes, encoded from the
+    // buffer values (decoding and encoding again need not give them back)
+    auto keep_received = [&]() {
+        if (unpack) {
+#pragma unroll
+            for (int k = 0; k < 5; k++) {
+                const int e = xplane(k);
+                if (odd)
+                    acc.store(e + 1, 0, P::store(u[k], e));
+                else
+                    acc.store(e, 0, P::store(u[k], e));
+            }
+        }
+    };
     if (flag == kGhostLayer) {   // packs its unchanged slots on a send plane
+        keep_received();
         xpack_alpha<P>(a, row, pl, pr, d);
         return;
     }
@@ -287,6 +382,7 @@ __device__ __forceinline__ void alpha_cell(const StepArgs<typename P::C>& a, uin
         }
     } else if (flag == kObstacle) {
         // no stores: the A-A layout makes bounce-back implicit (:330-343)
+        keep_received();
         if (pack)
             xpack_alpha<P>(a, row, pl, pr, d);
         if (a.store_u) {
@@ -314,14 +410,20 @@ __device__ __forceinline__ void alpha_cell(const StepArgs<typename P::C>& a, uin
 // what was read, so its stores are skipped. GHOST_LAYER passes values through.
 // Neighbour indices wrap modulo the cell count (domainWrap, :44-50). `flag`
 // is the cell's flag, as for alpha_cell.
-template <class P, class A>
+template <class P, int XF = kXfPack, class A>
 __device__ __forceinline__ void beta_cell(const StepArgs<typename P::C>& a, uint32_t id,
                                           int flag, const A& acc)
 {
     using C = typename P::C;
-    uint32_t row;
-    bool pl, pr;
-    const bool pack = xpack_where(a, id, row, pl, pr);
+    uint32_t row = 0;
+    bool pl = false, pr = false, ul = false, ur = false;
+    if (XF != kXfNone)
+        xpack_where(a, id, row, pl, pr, ul, ur);
+    if (XF != kXfUnpack)
+        ul = ur = false;
+    const bool pack = pl || pr;
+    const bool unpack = ul || ur;
+    const int odd = xunpack_odd<1>(ul);
     // plane l at id + c_l holds direction opp(l) arriving at this cell. As in
     // alpha_cell the slots are loaded whatever the flag (the caller's accessor
     // keeps every neighbour index in bounds); OBSTACLE, which writes back what
@@ -330,7 +432,22 @@ __device__ __forceinline__ void beta_cell(const StepArgs<typename P::C>& a, uint
 #pragma unroll
     for (int l = 0; l < kQ; l++)
         d[opp(l)] = P::load(acc.load(l, cx(l) + cy(l) * a.nx + cz(l) * a.nxy), l);
+    C u[5];
+    if (unpack)
+        xunpack_load<P, 1>(a, row, ul, odd, u);
     __builtin_amdgcn_sched_barrier(0);
+    if (unpack) {
+        // slot l's value is d[opp(l)]: slot e in d[e + 1], slot e + 1 in d[e]
+        uint32_t uov = 0;
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            const int e = xplane(k);
+            const C v = xunpack_value<P>(u[k], e, uov);
+            d[e + 1] = odd ? d[e + 1] : v;
+            d[e] = odd ? v : d[e];
+        }
+        raise_overflow<P>(a.status, uov);
+    }
 
     C rho, ux, uy, uz;
     moments(d, rho, ux, uy, uz);
@@ -415,6 +532,18 @@ __device__ __forceinline__ void beta_cell(const StepArgs<typename P::C>& a, uint
         d[17] = eq_a1(uz, vela2, rho);
         d[18] = eq_18(rho);
     } else if (flag == kObstacle) {
+        if (unpack) {
+            // no stores: the received values stay where the unpack puts them
+#pragma unroll
+            for (int k = 0; k < 5; k++) {
+                const int e = xplane(k);
+                if (odd)
+                    acc.store(e + 1, cx(e + 1) + cy(e + 1) * a.nx + cz(e + 1) * a.nxy,
+                              P::store(u[k], e));
+                else
+                    acc.store(e, cx(e) + cy(e) * a.nx + cz(e) * a.nxy, P::store(u[k], e));
+            }
+        }
         if (pack) {
             // the swap writes back the value read from plane l (:302-311)
             C out[kQ];

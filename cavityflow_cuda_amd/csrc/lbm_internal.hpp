@@ -61,7 +61,9 @@ struct StepArgs {
     const char* plane[19];
     // fused x-face pack: dense face buffers [slot][z * ny + y] (LEFT, RIGHT)
     C* xpack[2];
-    int xpack_on;
+    // fused x-face unpack: the received face buffers, same layout (LEFT, RIGHT)
+    const C* xunpack[2];
+    int xpack_on;     // any of xpack / xunpack set
     FastDiv div_nx;
     int32_t ny;
     int32_t fcells;   // ny * nz
@@ -90,6 +92,7 @@ struct lbm_solver {
     int launch[2];     // LBM_LAUNCH_* options of the alpha and beta kernels
     int launch_set[2]; // set by the caller: kept when the step pack changes
     void* xpack[2];    // fused x-face pack buffers (LEFT, RIGHT), NULL when off
+    const void* xunpack[2];  // fused x-face unpack buffers (LEFT, RIGHT), NULL when off
     void* dd_alt;      // second population buffer of the out-of-place pair pass
     uint32_t* status;  // device word: nonzero after a store out of the storage range
 };

@@ -156,6 +156,7 @@ __device__ __forceinline__ StepArgs<C> pair_args(const PairDesc& d)
     a.store_u = 0;
     a.xpack_on = 0;
     a.xpack[0] = a.xpack[1] = nullptr;
+    a.xunpack[0] = a.xunpack[1] = nullptr;
     a.status = d.status;
     return a;
 }
@@ -223,7 +224,7 @@ __global__ __launch_bounds__(G::kBlock) void k_pair(const PairDesc* __restrict__
         if (flag == kFluid || flag == kVelocityInjection) {
             const uint32_t id = (uint32_t)(((int64_t)p * ny + ay) * nx + ax);
             const PairAlphaAcc<S, kWin> acc{s0, lds, p, c};
-            alpha_cell<P>(a, id, flag, acc);
+            alpha_cell<P, kXfNone>(a, id, flag, acc);
         } else {
             // OBSTACLE / GHOST_LAYER: alpha leaves the slots as they are
 #pragma unroll
@@ -244,7 +245,7 @@ __global__ __launch_bounds__(G::kBlock) void k_pair(const PairDesc* __restrict__
         const int flag = lds_flag[((z + 3) % 3) * kWin + cw];
         if (flag != kObstacle) {
             const PairBetaAcc<S, NTS, kWin, kWX> acc{lds, z, cw, dst, stride, id};
-            beta_cell<P>(a, id, flag, acc);
+            beta_cell<P, kXfNone>(a, id, flag, acc);
             return;
         }
         // OBSTACLE: nothing overwrites the neighbours' post-alpha slots in place

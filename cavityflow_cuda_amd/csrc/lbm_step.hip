@@ -120,7 +120,7 @@ struct Shape {
     static constexpr uint32_t kSpan = kThreads * kCells;              // per workgroup
 };
 
-template <class P, bool FULL, bool OFF32, bool NT, int SHAPE>
+template <class P, bool FULL, bool OFF32, bool NT, int SHAPE, int XF>
 __global__ __launch_bounds__(Shape<SHAPE>::kThreads) void k_alpha(StepArgs<typename P::C> a)
 {
     using S = typename P::S;
@@ -133,15 +133,15 @@ __global__ __launch_bounds__(Shape<SHAPE>::kThreads) void k_alpha(StepArgs<typen
         const uint32_t id = FULL ? gid : box_cell(a, gid);
         if (OFF32) {
             const Acc32<S, NT> acc{a.plane, id * (uint32_t)sizeof(S)};
-            alpha_cell<P>(a, id, a.flags[id], acc);
+            alpha_cell<P, XF>(a, id, a.flags[id], acc);
         } else {
             const Acc64<S, false> acc{(S*)a.dd, a.stride, (int32_t)id, a.n};
-            alpha_cell<P>(a, id, a.flags[id], acc);
+            alpha_cell<P, XF>(a, id, a.flags[id], acc);
         }
     }
 }
 
-template <class P, bool FULL, bool OFF32, bool NT, int SHAPE>
+template <class P, bool FULL, bool OFF32, bool NT, int SHAPE, int XF>
 __global__ __launch_bounds__(Shape<SHAPE>::kThreads) void k_beta(StepArgs<typename P::C> a)
 {
     using S = typename P::S;
@@ -163,13 +163,13 @@ __global__ __launch_bounds__(Shape<SHAPE>::kThreads) void k_beta(StepArgs<typena
         const uint32_t id = FULL ? gid : box_cell(a, gid);
         if (wrap) {
             const Acc64<S, true> acc{(S*)a.dd, a.stride, (int32_t)id, a.n};
-            beta_cell<P>(a, id, a.flags[id], acc);
+            beta_cell<P, XF>(a, id, a.flags[id], acc);
         } else if (OFF32) {
             const Acc32<S, NT> acc{a.plane, id * (uint32_t)sizeof(S)};
-            beta_cell<P>(a, id, a.flags[id], acc);
+            beta_cell<P, XF>(a, id, a.flags[id], acc);
         } else {
             const Acc64<S, false> acc{(S*)a.dd, a.stride, (int32_t)id, a.n};
-            beta_cell<P>(a, id, a.flags[id], acc);
+            beta_cell<P, XF>(a, id, a.flags[id], acc);
         }
     }
 }
@@ -192,7 +192,7 @@ struct AccLds {
     __device__ __forceinline__ void store(int l, int32_t, S v) const { tile[l][t] = v; }
 };
 
-template <class P, bool NT>
+template <class P, bool NT, int XF>
 __global__ __launch_bounds__(kBlock) void k_beta_lds(StepArgs<typename P::C> a)
 {
     using S = typename P::S;
@@ -206,7 +206,7 @@ __global__ __launch_bounds__(kBlock) void k_beta_lds(StepArgs<typename P::C> a)
         const uint32_t id = b + t;
         if (id < a.cells) {
             const Acc64<S, true> acc{(S*)a.dd, a.stride, (int32_t)id, a.n};
-            beta_cell<P>(a, id, a.flags[id], acc);
+            beta_cell<P, XF>(a, id, a.flags[id], acc);
         }
         return;
     }
@@ -232,7 +232,7 @@ __global__ __launch_bounds__(kBlock) void k_beta_lds(StepArgs<typename P::C> a)
     }
     __syncthreads();
     const uint32_t id = b + t;
-    beta_cell<P>(a, id, a.flags[id], AccLds<S>{tile, t});
+    beta_cell<P, XF>(a, id, a.flags[id], AccLds<S>{tile, t});
     __syncthreads();
 #pragma unroll
     for (int l = 0; l < kQ; l++) {
@@ -284,7 +284,9 @@ static StepArgs<typename P::C> make_args(const lbm_solver* s, const Box& b)
     a.xcd = 0;
     a.xpack[0] = (C*)s->xpack[0];
     a.xpack[1] = (C*)s->xpack[1];
-    a.xpack_on = (s->xpack[0] || s->xpack[1]) ? 1 : 0;
+    a.xunpack[0] = (const C*)s->xunpack[0];
+    a.xunpack[1] = (const C*)s->xunpack[1];
+    a.xpack_on = (s->xpack[0] || s->xpack[1] || s->xunpack[0] || s->xunpack[1]) ? 1 : 0;
     a.div_nx = FastDiv((uint32_t)s->size_h[0]);
     a.ny = s->size_h[1];
     a.fcells = s->size_h[1] * s->size_h[2];
@@ -334,7 +336,8 @@ constexpr size_t kWaves2Lds = 54 * 1024;
 constexpr size_t kWideWaves4Lds = 54 * 1024;
 constexpr size_t kWideWaves2Lds = 81 * 1024;
 
-template <class P, template <class, bool, bool, bool, int> class K, bool FULL, int SHAPE>
+template <class P, template <class, bool, bool, bool, int, int> class K, bool FULL, int SHAPE,
+          int XF>
 static void launch_shape(const lbm_solver* s, const StepArgs<typename P::C>& a, int opts,
                          hipStream_t st)
 {
@@ -351,32 +354,52 @@ static void launch_shape(const lbm_solver* s, const StepArgs<typename P::C>& a, 
                        : (opts & LBM_LAUNCH_WAVES4) ? kWaves4Lds : 0;
     const dim3 grid(blocks), block(G::kThreads);
     if (!off32(s) || (opts & LBM_LAUNCH_ADDR64))
-        hipLaunchKernelGGL((K<P, FULL, false, false, SHAPE>::kernel), grid, block, lds, st, a);
+        hipLaunchKernelGGL((K<P, FULL, false, false, SHAPE, XF>::kernel), grid, block, lds, st, a);
     else if (opts & LBM_LAUNCH_NONTEMPORAL)
-        hipLaunchKernelGGL((K<P, FULL, true, true, SHAPE>::kernel), grid, block, lds, st, a);
+        hipLaunchKernelGGL((K<P, FULL, true, true, SHAPE, XF>::kernel), grid, block, lds, st, a);
     else
-        hipLaunchKernelGGL((K<P, FULL, true, false, SHAPE>::kernel), grid, block, lds, st, a);
+        hipLaunchKernelGGL((K<P, FULL, true, false, SHAPE, XF>::kernel), grid, block, lds, st, a);
 }
 
-template <class P, template <class, bool, bool, bool, int> class K, bool FULL>
+template <class P, template <class, bool, bool, bool, int, int> class K, bool FULL, int XF>
+static void launch_xf(const lbm_solver* s, const StepArgs<typename P::C>& a, int opts,
+                      hipStream_t st)
+{
+    if (opts & LBM_LAUNCH_BIG)
+        launch_shape<P, K, FULL, kShapeBig, XF>(s, a, opts, st);
+    else if (opts & LBM_LAUNCH_WIDE)
+        launch_shape<P, K, FULL, kShapeWide, XF>(s, a, opts, st);
+    else
+        launch_shape<P, K, FULL, kShapeBase, XF>(s, a, opts, st);
+}
+
+// The face work a solver's steps do: the fused unpack (with or without a
+// pack), the fused pack alone, or neither.
+static int face_work(const lbm_solver* s)
+{
+    return (s->xunpack[0] || s->xunpack[1]) ? kXfUnpack
+           : (s->xpack[0] || s->xpack[1])   ? kXfPack
+                                            : kXfNone;
+}
+
+template <class P, template <class, bool, bool, bool, int, int> class K, bool FULL>
 static void launch_variant(const lbm_solver* s, const StepArgs<typename P::C>& a, int opts,
                            hipStream_t st)
 {
-    if (opts & LBM_LAUNCH_BIG)
-        launch_shape<P, K, FULL, kShapeBig>(s, a, opts, st);
-    else if (opts & LBM_LAUNCH_WIDE)
-        launch_shape<P, K, FULL, kShapeWide>(s, a, opts, st);
-    else
-        launch_shape<P, K, FULL, kShapeBase>(s, a, opts, st);
+    switch (face_work(s)) {
+    case kXfUnpack: launch_xf<P, K, FULL, kXfUnpack>(s, a, opts, st); break;
+    case kXfPack: launch_xf<P, K, FULL, kXfPack>(s, a, opts, st); break;
+    default: launch_xf<P, K, FULL, kXfNone>(s, a, opts, st); break;
+    }
 }
 
-template <class P, bool FULL, bool OFF32, bool NT, int SHAPE>
+template <class P, bool FULL, bool OFF32, bool NT, int SHAPE, int XF>
 struct AlphaK {
-    static constexpr auto kernel = k_alpha<P, FULL, OFF32, NT, SHAPE>;
+    static constexpr auto kernel = k_alpha<P, FULL, OFF32, NT, SHAPE, XF>;
 };
-template <class P, bool FULL, bool OFF32, bool NT, int SHAPE>
+template <class P, bool FULL, bool OFF32, bool NT, int SHAPE, int XF>
 struct BetaK {
-    static constexpr auto kernel = k_beta<P, FULL, OFF32, NT, SHAPE>;
+    static constexpr auto kernel = k_beta<P, FULL, OFF32, NT, SHAPE, XF>;
 };
 
 template <class P>
@@ -393,8 +416,8 @@ static hipError_t alpha_t(const lbm_solver* s, const Box& b, hipStream_t st)
 
 // LBM_LAUNCH_LDS beta: the static tile counts towards the LDS reservation of
 // the waves-per-SIMD caps.
-template <class P>
-static void launch_beta_lds(const StepArgs<typename P::C>& a, int opts, hipStream_t st)
+template <class P, int XF>
+static void launch_beta_lds_xf(const StepArgs<typename P::C>& a, int opts, hipStream_t st)
 {
     constexpr size_t kTile = (size_t)kQ * kBlock * sizeof(typename P::S);
     const size_t want = (opts & LBM_LAUNCH_WAVES2)   ? kWaves2Lds
@@ -403,9 +426,20 @@ static void launch_beta_lds(const StepArgs<typename P::C>& a, int opts, hipStrea
     const size_t lds = want > kTile ? want - kTile : 0;
     const uint32_t blocks = (a.cells + kBlock - 1) / kBlock;
     if (opts & LBM_LAUNCH_NONTEMPORAL)
-        hipLaunchKernelGGL((k_beta_lds<P, true>), dim3(blocks), dim3(kBlock), lds, st, a);
+        hipLaunchKernelGGL((k_beta_lds<P, true, XF>), dim3(blocks), dim3(kBlock), lds, st, a);
     else
-        hipLaunchKernelGGL((k_beta_lds<P, false>), dim3(blocks), dim3(kBlock), lds, st, a);
+        hipLaunchKernelGGL((k_beta_lds<P, false, XF>), dim3(blocks), dim3(kBlock), lds, st, a);
+}
+
+template <class P>
+static void launch_beta_lds(const lbm_solver* s, const StepArgs<typename P::C>& a, int opts,
+                            hipStream_t st)
+{
+    switch (face_work(s)) {
+    case kXfUnpack: launch_beta_lds_xf<P, kXfUnpack>(a, opts, st); break;
+    case kXfPack: launch_beta_lds_xf<P, kXfPack>(a, opts, st); break;
+    default: launch_beta_lds_xf<P, kXfNone>(a, opts, st); break;
+    }
 }
 
 template <class P>
@@ -419,7 +453,7 @@ static hipError_t beta_t(const lbm_solver* s, const Box& b, hipStream_t st)
                       (int64_t)sizeof(typename P::S);
     const int opts = s->launch[1];
     if ((opts & LBM_LAUNCH_LDS) && is_full(s, b) && off32(s) && !(opts & LBM_LAUNCH_ADDR64)) {
-        launch_beta_lds<P>(a, opts, st);
+        launch_beta_lds<P>(s, a, opts, st);
         return hipGetLastError();
     }
     if (is_full(s, b))

@@ -258,6 +258,33 @@ class LbmSolverHIP:
             raise LbmError(_lib.LBM_OK - 1, "unpack_face needs a device buffer")
         check(lib().lbm_unpack_face(self._h, int(direction), int(phase), ptr, _stream(stream)))
 
+    def _step_buffers(self, left, right, what):
+        ptrs, keep = [], []
+        for b in (left, right):
+            if b is None:
+                ptrs.append(None)
+                continue
+            ptr, dev, k = _buffer(b)
+            if not dev:
+                raise LbmError(_lib.LBM_OK - 1, f"{what} needs device buffers")
+            ptrs.append(ptr)
+            keep.append(k)
+        return ptrs, keep
+
+    def set_step_pack(self, left=None, right=None):
+        """lbm_set_step_pack: the steps pack the LEFT / RIGHT x faces into
+        these device buffers (None: that side off)."""
+        (l, r), keep = self._step_buffers(left, right, "set_step_pack")
+        check(lib().lbm_set_step_pack(self._h, l, r))
+        self._step_pack_bufs = keep
+
+    def set_step_unpack(self, left=None, right=None):
+        """lbm_set_step_unpack: the steps take the x faces' received
+        populations from these device buffers (None: that side off)."""
+        (l, r), keep = self._step_buffers(left, right, "set_step_unpack")
+        check(lib().lbm_set_step_unpack(self._h, l, r))
+        self._step_unpack_bufs = keep
+
     def pack_face_pair(self, axis, phase, lo_buf, hi_buf, stream=None):
         """Both faces of one axis in one call: the low face (2 * axis) into
         lo_buf, the high face (2 * axis + 1) into hi_buf."""

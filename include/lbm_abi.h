@@ -331,6 +331,24 @@ int lbm_pack_face_pair(lbm_solver* solver, int axis, int phase, void* lo_buf, vo
  * No reference counterpart. */
 int lbm_set_step_pack(lbm_solver* solver, void* left_buf, void* right_buf);
 
+/* Fused halo unpacking of the two x faces from caller-owned device buffers
+ * in the layout lbm_pack_face writes (NULL disables a side, both NULL turns it
+ * off). While set, every whole-domain or box step takes the populations an x
+ * face receives from these buffers, with exactly the results (populations,
+ * rho / u, overflow flag) of running before it
+ *   alpha: lbm_unpack_face(solver, LBM_LEFT / LBM_RIGHT, 1, left / right_buf),
+ *   beta : lbm_unpack_face(solver, LBM_LEFT / LBM_RIGHT, 0, left / right_buf);
+ * the caller skips those x-face unpacks. The send-plane cells read the
+ * received values instead of the slots the unpack would have written (cells
+ * that store nothing write them there). Each buffer must hold the neighbour's
+ * pack of the previous half-step when a step runs. A phase-0 and a phase-1
+ * pack of a face carry disjoint planes, so a neighbour's lbm_set_step_pack
+ * buffer (same device) can serve as the receive buffer directly, with no
+ * copy. LBM_ERR_UNSUPPORTED for an x extent below 2 interior cells. No
+ * reference counterpart: the reference unpacks with setDensityDistributions
+ * (CController.cpp:410-436, 848-874). */
+int lbm_set_step_unpack(lbm_solver* solver, const void* left_buf, const void* right_buf);
+
 /* Single-GPU stand-in for the neighbour send/recv of one face: packs `from`'s
  * face `direction` (phase as in lbm_pack_face) through `from`'s device
  * staging buffer and unpacks it into `to` across the opposite face, e.g.
