@@ -10,7 +10,7 @@ the cavity walls stay physical; for FP32 the x faces are packed and unpacked ins
 the step kernels (lbm_set_step_pack / lbm_set_step_unpack), each face receiving its
 own pack of the step pair before -- a wall reflected through the exchange path, which
 leaves the cavity unchanged -- from one of two buffers while packing into the
-other (for the 16-bit formats the x faces use the copy kernels too).
+other (also for F16S and BF16S; F16C and FP64 use the copy kernels for x too).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 512]
                     [--precision f32|f64|f16s|bf16s|f16c] [--no-halo]
@@ -488,10 +488,11 @@ def step_run(args, cf, L, hip, stream, precision, steps, warmup):
     # the exchange path (the pack stores what the wall's bounce-back returns,
     # so the cavity is unchanged), with the bytes and access pattern of a
     # neighbour's exchange -- and no x-face copy kernel runs
-    # measured at 512^3: fused 33.8 k vs 33.6 k MLUPS FP32; the 16-bit betas
-    # lose 14-22 % with the received loads (F16S 57.2 k vs 60.3 k)
+    # measured at 512^3: FP32 34.3 k MLUPS fused (alpha in 256-thread
+    # workgroups at 3 waves per SIMD) vs 33.7 k with the copy kernels; F16S
+    # 60.9 k (beta in 256-thread workgroups) vs 60.2 k; F16C 59.1 k vs 60.4 k
     x_mode = args.x_exchange if args.x_exchange != "auto" else (
-        "fused" if precision == "f32" else "kernels")
+        "fused" if precision in ("f32", "f16s", "bf16s") else "kernels")
     x_in_steps = fused and x_mode == "fused"
     xbufs = []
     if x_in_steps:
@@ -508,7 +509,8 @@ def step_run(args, cf, L, hip, stream, precision, steps, warmup):
         for d in (0, 1):
             for phase in (0, 1):
                 check(L.lbm_pack_face(solver.handle, d, phase, xbufs[1][d], stream))
-    # as launched (defaults may depend on the step pack)
+        check(L.lbm_set_step_unpack(solver.handle, xbufs[1][0], xbufs[1][1]))
+    # as launched (defaults may depend on the step pack and unpack)
     launch_opts = []
     for k in (0, 1):
         v = ctypes.c_int()
@@ -848,7 +850,8 @@ def main():
                     help="x faces: packed and unpacked inside the step kernels (fused; each "
                          "face receives its own pack, a reflected wall) or unpacked by "
                          "lbm_unpack_face into the shadow subdomain (kernels); auto: fused "
-                         "for FP32, kernels for the other formats (measured at 512^3)")
+                         "for FP32, F16S and BF16S, kernels for F16C and FP64 (measured "
+                         "at 512^3)")
     ap.add_argument("--no-fused-pack", action="store_true",
                     help="pack the x faces with separate kernels instead of inside the steps")
     ap.add_argument("--cpu-sample-steps", type=int, default=2)

@@ -823,6 +823,18 @@ int lbm_pack_face_pair(lbm_solver* s, int axis, int phase, void* lo_buf, void* h
     return LBM_OK;
 }
 
+// Launch options the caller has not set follow the precision's defaults for
+// the solver's size and fused face work.
+static void refresh_launch_defaults(lbm_solver* s)
+{
+    const bool with_pack = s->xpack[0] || s->xpack[1];
+    const bool with_unpack = s->xunpack[0] || s->xunpack[1];
+    for (int k = 0; k < 2; k++)
+        if (!s->launch_set[k])
+            s->launch[k] = default_launch(s->p.precision, k, with_pack, s->n >= kWideCells,
+                                          with_unpack);
+}
+
 int lbm_set_step_pack(lbm_solver* s, void* left_buf, void* right_buf)
 {
     if (!s) {
@@ -831,10 +843,7 @@ int lbm_set_step_pack(lbm_solver* s, void* left_buf, void* right_buf)
     }
     s->xpack[0] = left_buf;
     s->xpack[1] = right_buf;
-    const bool with_pack = left_buf || right_buf;
-    for (int k = 0; k < 2; k++)
-        if (!s->launch_set[k])
-            s->launch[k] = default_launch(s->p.precision, k, with_pack, s->n >= kWideCells);
+    refresh_launch_defaults(s);
     return LBM_OK;
 }
 
@@ -851,6 +860,7 @@ int lbm_set_step_unpack(lbm_solver* s, const void* left_buf, const void* right_b
     }
     s->xunpack[0] = left_buf;
     s->xunpack[1] = right_buf;
+    refresh_launch_defaults(s);
     return LBM_OK;
 }
 

@@ -172,16 +172,20 @@ hipError_t launch_reduce_boxes(const lbm_solver* const* s, const Box* b, int cou
 // (F16C beta, with more integer work per cell, loses with them).
 constexpr int64_t kWideCells = 6 << 20;   // halo-inclusive cells for `wide`
 
-inline int default_launch(int precision, int kernel, bool with_pack, bool wide)
+inline int default_launch(int precision, int kernel, bool with_pack, bool wide,
+                          bool with_unpack = false)
 {
     switch (precision) {
     case LBM_F32:
         if (kernel == 0) {
-            if (with_pack && !wide)
+            // 256-thread workgroups, 3 waves per SIMD: with a step pack on a
+            // small domain, and with the fused x-face unpack (whose received
+            // loads hold a wave longer than the streaming 2-wave shape allows)
+            if (with_unpack || (with_pack && !wide))
                 return LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_WAVES3;
             return (wide ? LBM_LAUNCH_WIDE : 0) | LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_WAVES2;
         }
-        if (with_pack && wide)
+        if ((with_pack || with_unpack) && wide)
             return LBM_LAUNCH_BIG | LBM_LAUNCH_WAVES4;
         return (wide ? LBM_LAUNCH_WIDE : 0) | LBM_LAUNCH_WAVES2;
     case LBM_F64:
@@ -190,7 +194,9 @@ inline int default_launch(int precision, int kernel, bool with_pack, bool wide)
     case LBM_F16C:
         return 0;
     default:
-        return kernel == 1 && wide ? LBM_LAUNCH_BIG : 0;
+        // 1024-thread beta workgroups wait on their slowest wave: with the
+        // fused unpack's received loads 256-thread ones are faster
+        return kernel == 1 && wide && !with_unpack ? LBM_LAUNCH_BIG : 0;
     }
 }
 

@@ -175,10 +175,12 @@ int lbm_simulation_step_beta(lbm_solver* solver, const int origin[3], const int 
 /* Launch variants of the step kernels; they change speed, never results.
  * kernel: 0 = alpha, 1 = beta. Defaults (measured on MI355X): F32 beta
  * WIDE | WAVES2 on domains of 6 Mi cells or more (halo included: from 183^3
- * interior cells), BIG | WAVES4 there with a step pack (lbm_set_step_pack),
- * and WAVES2 below; F32 alpha NONTEMPORAL | WAVES2 plus WIDE on those large
- * domains, NONTEMPORAL | WAVES3 with a step pack on smaller ones; F64 alpha NONTEMPORAL | WAVES2, beta LDS | WAVES2; F16S /
- * BF16S alpha none, beta BIG on those large domains (else none); F16C none.
+ * interior cells), BIG | WAVES4 there with a step pack or unpack
+ * (lbm_set_step_pack / lbm_set_step_unpack), and WAVES2 below; F32 alpha
+ * NONTEMPORAL | WAVES2 plus WIDE on those large domains, NONTEMPORAL | WAVES3
+ * with a step unpack, or with a step pack on smaller domains; F64 alpha
+ * NONTEMPORAL | WAVES2, beta LDS | WAVES2; F16S / BF16S alpha none, beta BIG
+ * on those large domains without a step unpack (else none); F16C none.
  * Batches start from F32 / F64 alpha and beta as for 256-thread workgroups
  * with a step pack, 16-bit alpha NONTEMPORAL and beta none. No reference
  * counterpart: the reference takes its block shapes from the XML

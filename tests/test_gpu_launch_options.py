@@ -97,7 +97,8 @@ def test_default_options_are_valid_and_settable(hip, precision):
 def test_large_domains_default_to_bigger_workgroups(hip, precision, small, large):
     """From 6 Mi cells (halo included) FP32 defaults to 512-thread
     workgroups and F16S / BF16S beta to 1024-thread workgroups; with a step
-    pack FP32 beta there takes 1024-thread workgroups at 4 waves per SIMD."""
+    pack FP32 beta there takes 1024-thread workgroups at 4 waves per SIMD,
+    and with the fused unpack FP32 alpha 256-thread workgroups at 3 waves."""
     import ctypes
     from cavityflow_cuda_amd._lib import check, lib
     v = ctypes.c_int()
@@ -107,6 +108,17 @@ def test_large_domains_default_to_bigger_workgroups(hip, precision, small, large
         for k in (0, 1):
             check(lib().lbm_get_launch_options(s.handle, k, ctypes.byref(v)))
             assert v.value == want[k], (size, k)
+        if precision in ("f16s", "bf16s") and size[0] == 183:
+            # with the fused unpack, beta in 256-thread workgroups
+            buf = ctypes.c_void_p()
+            check(lib().lbm_device_alloc(0, s.face_elements(0) * 4, ctypes.byref(buf)))
+            check(lib().lbm_set_step_unpack(s.handle, buf, buf))
+            check(lib().lbm_get_launch_options(s.handle, 1, ctypes.byref(v)))
+            assert v.value == 0
+            check(lib().lbm_set_step_unpack(s.handle, None, None))
+            check(lib().lbm_get_launch_options(s.handle, 1, ctypes.byref(v)))
+            assert v.value == 256
+            check(lib().lbm_device_free(0, buf))
         if precision == "f32" and size[0] == 183:
             buf = ctypes.c_void_p()
             check(lib().lbm_device_alloc(0, s.face_elements(0) * 4, ctypes.byref(buf)))
@@ -114,7 +126,17 @@ def test_large_domains_default_to_bigger_workgroups(hip, precision, small, large
             for k, w in ((0, 64 | 1 | 16), (1, 256 | 4)):
                 check(lib().lbm_get_launch_options(s.handle, k, ctypes.byref(v)))
                 assert v.value == w, ("pack", k)
+            # with the fused unpack as well, alpha takes 256-thread workgroups
+            # at 3 waves per SIMD; both off, the first defaults return
+            check(lib().lbm_set_step_unpack(s.handle, buf, buf))
+            for k, w in ((0, 1 | 8), (1, 256 | 4)):
+                check(lib().lbm_get_launch_options(s.handle, k, ctypes.byref(v)))
+                assert v.value == w, ("unpack", k)
+            check(lib().lbm_set_step_unpack(s.handle, None, None))
             check(lib().lbm_set_step_pack(s.handle, None, None))
+            for k in (0, 1):
+                check(lib().lbm_get_launch_options(s.handle, k, ctypes.byref(v)))
+                assert v.value == large[k], ("off", k)
             check(lib().lbm_device_free(0, buf))
         s.close()
 
