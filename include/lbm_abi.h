@@ -346,7 +346,13 @@ int lbm_set_step_pack(lbm_solver* solver, void* left_buf, void* right_buf);
  * pack of the previous half-step when a step runs. A phase-0 and a phase-1
  * pack of a face carry disjoint planes, so a neighbour's lbm_set_step_pack
  * buffer (same device) can serve as the receive buffer directly, with no
- * copy. LBM_ERR_UNSUPPORTED for an x extent below 2 interior cells. No
+ * copy. Ordering: the reference unpacks x before it packs y and z, whose face
+ * boxes include the x-halo cells, so in a decomposition also split along y
+ * or z its y / z exchange carries the received x values on to the edge cells
+ * of those neighbours; a fused unpack applies them only inside the next
+ * step, after such exchanges. It reproduces the reference order where x is
+ * the only split axis. LBM_ERR_UNSUPPORTED for an x extent below 2 interior
+ * cells. No
  * reference counterpart: the reference unpacks with setDensityDistributions
  * (CController.cpp:410-436, 848-874). */
 int lbm_set_step_unpack(lbm_solver* solver, const void* left_buf, const void* right_buf);
