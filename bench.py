@@ -782,19 +782,23 @@ def split_workload(args, cf, L, hip, stream):
     `split`), faces exchanged on the device after every half-step: the serial
     order (whole subdomains, then the exchange) against the reference's
     boundary-first order with the exchange on a second stream
-    (overlap=True); both give the same bits."""
+    (overlap=True) and, for a split along x only, the x faces exchanged inside
+    the step kernels (fused_x=True); all give the same bits."""
     from cavityflow_cuda_amd import _hip
     size = (args.size,) * 3
     split = tuple(args.split)
     params = cf.cavity_parameters(size, args.u_lid, args.reynolds, args.precision)
     cells = int(np.prod(size))
     runs = {}
-    variants = ((False, 2), (True, 2), (True, 32)) if split[0] > 1 else ((False, 2), (True, 2))
-    for overlap, xslab in variants:
+    variants = [("serial", {}), ("overlap_xslab2", dict(overlap=True, xslab=2))]
+    if split[0] > 1:
+        variants.append(("overlap_xslab32", dict(overlap=True, xslab=32)))
+    if split[0] > 1 and split[1] == split[2] == 1:
+        variants.append(("fused_x", dict(fused_x=True)))
+    for key, kw in variants:
         main_stream = _hip.Stream(0)
         c = cf.CavityController(size, precision=args.precision, store_outputs=False,
-                                split=split, params=params, stream=main_stream.handle,
-                                overlap=overlap, xslab=xslab)
+                                split=split, params=params, stream=main_stream.handle, **kw)
         st = ctypes.c_void_p(main_stream.handle)
         for _ in range(args.warmup):
             c.computeNextStep()
@@ -805,7 +809,6 @@ def split_workload(args, cf, L, hip, stream):
             c.computeNextStep()
         hip.record(e1, st)
         ms = hip.elapsed_ms(e0, e1)
-        key = f"overlap_xslab{xslab}" if overlap else "serial"
         runs[key] = dict(mlups=round(cells * args.steps / (ms * 1e-3) / 1e6, 1),
                          ms_per_step=round(ms / args.steps, 4))
         print(f"[split] {key} {runs[key]}", file=sys.stderr, flush=True)

@@ -21,6 +21,15 @@ surface of the reference, on one GPU.
   inner box, and the faces are exchanged on a second stream while the inner
   boxes run. The inner boxes touch neither the planes the exchange reads nor
   the slots it writes, so the result is bit-identical to the serial order.
+* ``fused_x=True`` (a split along x only) exchanges the x faces inside the
+  step kernels: each subdomain packs its x faces into its own buffers
+  (``lbm_set_step_pack``) and its steps read the neighbours' packs as the
+  received faces (``lbm_set_step_unpack``), so no x-face copy kernel runs.
+  After alpha and after beta the packs carry disjoint planes, so one buffer
+  per face serves both; the result is bit-identical to the serial order.
+  With y or z splits as well the reference forwards received x values to the
+  y / z neighbours' edge cells through their exchange, which would run before
+  a fused unpack, so those decompositions keep ``lbm_exchange_face``.
 * ``run`` times ``loops`` steps and reports the reference's benchmark figures
   (CController.cpp:1382-1490): GLUPS over interior cells and its
   "bandwidth" = GLUPS * (19 * 2 * sizeof(T) + sizeof(Flag) [+ 4 * sizeof(T)]).
@@ -53,13 +62,15 @@ class CavityController:
 
     def __init__(self, size, u_lid=0.05, reynolds=100.0, precision="f32",
                  store_outputs=True, split=(1, 1, 1), device=0, params=None, stream=None,
-                 overlap=False, xslab=2):
+                 overlap=False, xslab=2, fused_x=False):
         self.size = tuple(int(v) for v in size)
         self.split = tuple(int(v) for v in split)
         for a in range(3):
             if self.size[a] % self.split[a]:
                 raise ValueError("domain size must be divisible by the split")
         self.sub = tuple(self.size[a] // self.split[a] for a in range(3))
+        if fused_x and (overlap or self.split[0] < 2 or self.split[1] != 1 or self.split[2] != 1):
+            raise ValueError("fused_x needs a split along x only, without overlap")
         self.precision = precision_code(precision)
         self.dtype = host_dtype(precision)
         self.params = params or cavity_parameters(self.size, u_lid, reynolds, precision)
@@ -113,6 +124,13 @@ class CavityController:
             self._slabs_done = Event()
             self._exchange_done = Event()
             self._boxes = {key: self._slab_boxes(key) for key in self.solvers}
+        self.device = device
+        self.fused_x = bool(fused_x)
+        self._xbufs = []
+        self._x_recv = {}
+        self._x_armed = False
+        if self.fused_x:
+            self._setup_fused_x()
 
     @classmethod
     def from_config(cls, cfg, device=0, stream=None, vis_format="netcdf") -> "CavityController":
@@ -159,10 +177,51 @@ class CavityController:
         return [p for key, w in self.writers.items()
                 if (p := w.render(iteration, self.solvers[key])) is not None]
 
+    def _setup_fused_x(self):
+        """One pack buffer per communicating x face; a subdomain's received
+        LEFT face is its left neighbour's RIGHT pack and vice versa."""
+        import ctypes
+        L = lib()
+        elem = np.dtype(self.dtype).itemsize
+        pack = {}
+        for key, s in self.solvers.items():
+            bufs = [None, None]
+            for d in (0, 1):
+                nb = (key[0] + (1 if d else -1), key[1], key[2])
+                if nb in self.solvers:
+                    p = ctypes.c_void_p()
+                    check(L.lbm_device_alloc(self.device, s.face_elements(d) * elem,
+                                             ctypes.byref(p)))
+                    self._xbufs.append(p)
+                    bufs[d] = p
+            pack[key] = bufs
+            check(L.lbm_set_step_pack(s.handle, bufs[0], bufs[1]))
+        for key in self.solvers:
+            left = pack.get((key[0] - 1, key[1], key[2]))
+            right = pack.get((key[0] + 1, key[1], key[2]))
+            self._x_recv[key] = (left[1] if left else None, right[0] if right else None)
+
+    def _arm_fused_x(self):
+        # from the first beta on: before it nothing was exchanged, and the
+        # first alpha reads the subdomains' own slots
+        L = lib()
+        for key, s in self.solvers.items():
+            check(L.lbm_set_step_unpack(s.handle, *self._x_recv[key]))
+        self._x_armed = True
+
     def close(self):
         for st in self._streams:
             st.close()
         self._streams = []
+        if self._xbufs:
+            L = lib()
+            self.sync()
+            for s in self.solvers.values():
+                check(L.lbm_set_step_unpack(s.handle, None, None))
+                check(L.lbm_set_step_pack(s.handle, None, None))
+            for p in self._xbufs:
+                L.lbm_device_free(self.device, p)
+            self._xbufs = []
         if self.overlap:
             self._slabs_done.close()
             self._exchange_done.close()
@@ -236,15 +295,20 @@ class CavityController:
             return self._step_overlapped(0)
         for s in self.solvers.values():
             s.simulationStepAlpha(stream=self.stream)
-        if len(self.solvers) > 1:
+        if self.fused_x:
+            if not self._x_armed:
+                self._arm_fused_x()
+        elif len(self.solvers) > 1:
             self._exchange(0)
 
     def stepBeta(self):
         if self.overlap:
             return self._step_overlapped(1)
+        if self.fused_x and not self._x_armed:
+            raise RuntimeError("fused_x: the first step must be an alpha step")
         for s in self.solvers.values():
             s.simulationStepBeta(stream=self.stream)
-        if len(self.solvers) > 1:
+        if not self.fused_x and len(self.solvers) > 1:
             self._exchange(1)
 
     def computeNextStep(self):

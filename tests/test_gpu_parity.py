@@ -187,6 +187,33 @@ def test_overlapped_exchange_equals_serial_order(hip, split, size, prec):
     assert runs["overlap"] == runs["overlap_x8"] == runs["serial"] == runs["mono"]
 
 
+@pytest.mark.parametrize("split,size", [((2, 1, 1), (24, 20, 18)), ((3, 1, 1), (30, 14, 16)),
+                                        ((4, 1, 1), (8, 6, 7))])
+@pytest.mark.parametrize("prec", ["f32", "f16s", "f64"])
+def test_fused_x_exchange_equals_serial_order(hip, split, size, prec):
+    """Subdomains split along x exchanging their x faces inside the step
+    kernels (each reads its neighbours' fused-pack buffers, no copy kernel)
+    equal the serial order with lbm_exchange_face and the undivided cavity,
+    bit for bit; the smallest case has subdomains of 2 cells."""
+    say("fused x exchange", split, size, prec)
+    p = hip.cavity_parameters(size, 0.08, 10.0 * max(size), prec)
+    runs = {}
+    for name, kw in (("mono", dict()), ("serial", dict(split=split)),
+                     ("fused", dict(split=split, fused_x=True))):
+        c = hip.CavityController(size, precision=prec, params=p, **kw)
+        for _ in range(41):
+            c.computeNextStep()
+        runs[name] = (c.getVelocities().tobytes(), c.getDensities().tobytes())
+        c.close()
+    assert runs["fused"] == runs["serial"] == runs["mono"]
+
+
+def test_fused_x_exchange_rejects_other_splits(hip):
+    for kw in (dict(split=(2, 2, 1)), dict(split=(1, 1, 2)), dict(split=(2, 1, 1), overlap=True)):
+        with pytest.raises(ValueError):
+            hip.CavityController((8, 8, 8), fused_x=True, **kw)
+
+
 def test_overlap_rejects_subdomains_too_thin(hip):
     with pytest.raises(ValueError):
         hip.CavityController((2, 8, 8), split=(2, 1, 1), overlap=True)
