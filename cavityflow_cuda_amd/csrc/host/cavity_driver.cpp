@@ -4,6 +4,7 @@
 //
 //   cavity_driver --size NX NY NZ [--split PX PY PZ] [--u-lid U] [--re RE]
 //                 [--loops N] [--precision f32|f64|f16s|bf16s|f16c] [--dump FILE]
+//                 [--fused-x]
 //   cavity_driver --config FILE.xml [--precision ...] [--loops N] [--dump FILE]
 //                 [--vis-format netcdf|netcdf-global|vtk]
 //   cavity_driver --config FILE.xml --print-config
@@ -32,7 +33,10 @@
 //   the lid goes on the top layer of the top subdomains (:206-208);
 // * computeNextStep alternates alpha / beta by step parity (:1278-1287) and
 //   exchanges faces after each half-step in the order x, y, z
-//   (CController.cpp:369-570, 807-1008), on the device;
+//   (CController.cpp:369-570, 807-1008), on the device; with --fused-x (a
+//   split along x only) the x faces are exchanged inside the step kernels
+//   instead, each subdomain reading its neighbours' fused-pack buffers (the
+//   same bits, no copy kernel);
 // * the benchmark figures are the reference's (CController.cpp:1382-1490):
 //   GLUPS over interior cells and GBytes/s = GLUPS * bytes per cell (19 * 2
 //   populations + the flag + 4 outputs when stored); with benchmark on in
@@ -52,6 +56,8 @@
 #include <string>
 #include <vector>
 
+#include <array>
+
 #include "CLbmSolverHIP.hpp"
 #include "lbm_config.hpp"
 #include "lbm_netcdf.hpp"
@@ -70,6 +76,7 @@ struct Options {
     std::string vis_format = "netcdf";
     bool print_config = false;
     bool loops_set = false;
+    bool fused_x = false;   // x faces exchanged inside the steps (x-only splits)
 };
 
 int precision_code(const std::string& p)
@@ -147,7 +154,25 @@ public:
                     if (j == o.split[1] - 1)
                         solvers_.back()->setDrivenCavitySzenario();
                 }
+        if (o.fused_x)
+            setup_fused_x();
     }
+
+    ~Controller()
+    {
+        if (xbufs_.empty())
+            return;
+        (void)lbm_sync(solvers_[0]->handle(), nullptr);
+        for (auto& s : solvers_) {
+            (void)lbm_set_step_unpack(s->handle(), nullptr, nullptr);
+            (void)lbm_set_step_pack(s->handle(), nullptr, nullptr);
+        }
+        for (void* p : xbufs_)
+            (void)lbm_device_free(0, p);
+    }
+
+    Controller(const Controller&) = delete;
+    Controller& operator=(const Controller&) = delete;
 
     const lbm::SolverParameters& params() const { return params_; }
 
@@ -175,17 +200,57 @@ public:
                     }
     }
 
+    // x faces exchanged inside the step kernels: every subdomain packs its
+    // communicating x faces into buffers of its own (lbm_set_step_pack) and
+    // its steps read the neighbours' packs as received faces
+    // (lbm_set_step_unpack); the packs after alpha and after beta carry
+    // disjoint planes, so one buffer per face serves both. Exact for splits
+    // along x only (with y / z splits the reference's y / z exchange forwards
+    // received x values to edge cells before a fused unpack would apply them).
+    void setup_fused_x()
+    {
+        if (o_.split[0] < 2 || o_.split[1] != 1 || o_.split[2] != 1)
+            throw lbm::Error(LBM_ERR_ARG, "--fused-x needs a split along x only");
+        const int n = o_.split[0];
+        std::vector<void*> left(n, nullptr), right(n, nullptr);
+        for (int i = 0; i < n; i++) {
+            lbm_solver* h = at(i, 0, 0)->handle();
+            for (int d = 0; d < 2; d++) {
+                if ((d == 0 && i == 0) || (d == 1 && i == n - 1))
+                    continue;
+                void* p = nullptr;
+                lbm::check(lbm_device_alloc(0, (size_t)lbm_face_elements(h, d) * sizeof(T), &p));
+                xbufs_.push_back(p);
+                (d ? right : left)[i] = p;
+            }
+            lbm::check(lbm_set_step_pack(h, left[i], right[i]));
+        }
+        for (int i = 0; i < n; i++)
+            xrecv_.push_back({i > 0 ? right[i - 1] : nullptr, i < n - 1 ? left[i + 1] : nullptr});
+    }
+
     void computeNextStep()
     {
         const int phase = counter_ & 1;
+        if (phase && !xrecv_.empty() && !xarmed_)
+            throw lbm::Error(LBM_ERR_ARG, "--fused-x: the first step must be an alpha step");
         for (auto& s : solvers_) {
             if (phase)
                 s->simulationStepBeta();
             else
                 s->simulationStepAlpha();
         }
-        if (solvers_.size() > 1)
+        if (!xrecv_.empty()) {
+            // from the first beta on: before it nothing was exchanged
+            if (!xarmed_) {
+                for (size_t i = 0; i < solvers_.size(); i++)
+                    lbm::check(lbm_set_step_unpack(solvers_[i]->handle(), xrecv_[i][0],
+                                                   xrecv_[i][1]));
+                xarmed_ = true;
+            }
+        } else if (solvers_.size() > 1) {
             exchange(phase);
+        }
         counter_++;
     }
 
@@ -292,6 +357,9 @@ private:
     int sub_[3];
     lbm::SolverParameters params_;
     std::vector<std::unique_ptr<lbm::CLbmSolverHIP<T>>> solvers_;
+    std::vector<void*> xbufs_;                  // fused x exchange: pack buffers
+    std::vector<std::array<void*, 2>> xrecv_;   // per solver: received LEFT, RIGHT
+    bool xarmed_ = false;
     long counter_ = 0;
 };
 
@@ -460,6 +528,8 @@ int main(int argc, char** argv)
             o.config = argv[++i];
         } else if (a == "--print-config") {
             o.print_config = true;
+        } else if (a == "--fused-x") {
+            o.fused_x = true;
         } else if (a == "--precision") {
             need(1);
             o.precision = argv[++i];

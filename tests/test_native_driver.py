@@ -19,14 +19,18 @@ def test_driver_is_built_and_links():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("prec,dtype", [("f32", np.float32), ("f64", np.float64)])
-@pytest.mark.parametrize("split", [(1, 1, 1), (2, 1, 2)])
-def test_driver_matches_oracle(oracle_cpu, tmp_path, prec, dtype, split):
+@pytest.mark.parametrize("split,fused_x", [((1, 1, 1), False), ((2, 1, 2), False),
+                                           ((5, 1, 1), True)])
+def test_driver_matches_oracle(oracle_cpu, tmp_path, prec, dtype, split, fused_x):
+    """Undivided, split with the face exchange after each half-step, and
+    split along x with the x faces exchanged inside the step kernels
+    (--fused-x): the reference CPU kernels' cavity, bit for bit."""
     size, loops = (20, 16, 12), 41
-    print("[case] native driver", prec, split, flush=True)
+    print("[case] native driver", prec, split, "fused x" if fused_x else "", flush=True)
     dump = str(tmp_path / "state.bin")
     p = subprocess.run([DRIVER, "--size", *map(str, size), "--split", *map(str, split),
                         "--u-lid", "0.05", "--re", "100", "--loops", str(loops),
-                        "--precision", prec, "--dump", dump],
+                        "--precision", prec, "--dump", dump] + (["--fused-x"] if fused_x else []),
                        capture_output=True, text=True, timeout=120)
     assert p.returncode == 0, p.stderr
     assert "GLUPS" in p.stdout
