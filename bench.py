@@ -490,7 +490,8 @@ def step_run(args, cf, L, hip, stream, precision, steps, warmup):
     # neighbour's exchange -- and no x-face copy kernel runs
     # measured at 512^3: FP32 34.3 k MLUPS fused (alpha in 256-thread
     # workgroups at 3 waves per SIMD) vs 33.7 k with the copy kernels; F16S
-    # 60.9 k (beta in 256-thread workgroups) vs 60.2 k; F16C 59.1 k vs 60.4 k
+    # 60.9 k (beta in 256-thread workgroups) vs 60.2 k; F16C 59.1 k vs 60.4 k;
+    # FP64 17.1 k vs 17.6 k
     x_mode = args.x_exchange if args.x_exchange != "auto" else (
         "fused" if precision in ("f32", "f16s", "bf16s") else "kernels")
     x_in_steps = fused and x_mode == "fused"
