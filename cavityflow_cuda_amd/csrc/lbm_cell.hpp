@@ -174,6 +174,9 @@ constexpr int kXfNone = 0, kXfPack = 1, kXfUnpack = 2;
 // planes of a pair decode with xplane(k).
 __host__ __device__ constexpr int xplane(int k) { return k == 0 ? 0 : 2 * k + 2; }
 
+// The plane of each pair a face cell receives: before alpha (STEP 0) the
+// LEFT face brings c_x = +1 and the RIGHT face c_x = -1, before beta the
+// reverse.
 template <int STEP>
 __device__ __forceinline__ int xunpack_odd(bool ul)
 {
