@@ -223,7 +223,7 @@ public:
                 xbufs_.push_back(p);
                 (d ? right : left)[i] = p;
             }
-            lbm::check(lbm_set_step_pack(h, left[i], right[i]));
+            at(i, 0, 0)->setStepPack(static_cast<T*>(left[i]), static_cast<T*>(right[i]));
         }
         for (int i = 0; i < n; i++)
             xrecv_.push_back({i > 0 ? right[i - 1] : nullptr, i < n - 1 ? left[i + 1] : nullptr});
@@ -244,8 +244,8 @@ public:
             // from the first beta on: before it nothing was exchanged
             if (!xarmed_) {
                 for (size_t i = 0; i < solvers_.size(); i++)
-                    lbm::check(lbm_set_step_unpack(solvers_[i]->handle(), xrecv_[i][0],
-                                                   xrecv_[i][1]));
+                    solvers_[i]->setStepUnpack(static_cast<const T*>(xrecv_[i][0]),
+                                               static_cast<const T*>(xrecv_[i][1]));
                 xarmed_ = true;
             }
         } else if (solvers_.size() > 1) {

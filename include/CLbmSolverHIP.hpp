@@ -178,6 +178,18 @@ public:
                                                 1, stream));
     }
 
+    // The x-face exchange inside the step kernels (no reference counterpart):
+    // device buffers of lbm_face_elements(LBM_LEFT) values of T each, nullptr
+    // for a side without a neighbour. setStepPack: the steps pack the 5
+    // populations per face and phase that lbm_pack_face packs; setStepUnpack:
+    // the steps take the received ones, with the results of lbm_unpack_face
+    // before each step (exact where x is the only split axis).
+    void setStepPack(T* left, T* right) { check(lbm_set_step_pack(s_, left, right)); }
+    void setStepUnpack(const T* left, const T* right)
+    {
+        check(lbm_set_step_unpack(s_, left, right));
+    }
+
     // get/setFlags (CLbmSolver.hpp:77-80)
     void getFlags(Vec3i& origin, Vec3i& size, Flag* dst)
     {
