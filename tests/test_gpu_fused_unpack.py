@@ -15,8 +15,9 @@ OBST, FLUID, INJ, GHOST = 1, 2, 4, 8
 LEFT, RIGHT = 0, 1
 W = np.array([1 / 18] * 4 + [1 / 36] * 12 + [1 / 18] * 2 + [1 / 3])
 # alpha / beta launch options: default, base 256 threads, WIDE + nontemporal +
-# 2 waves, BIG + 4 waves, XCD map, and LDS-staged beta
-OPTIONS = [None, (0, 0), (64 | 1 | 16, 64 | 16), (256 | 4, 256 | 4), (2 | 8, 2 | 4), (1, 128 | 4)]
+# 2 waves, BIG + 4 waves, XCD map, LDS-staged beta, 64-bit element indexing
+OPTIONS = [None, (0, 0), (64 | 1 | 16, 64 | 16), (256 | 4, 256 | 4), (2 | 8, 2 | 4), (1, 128 | 4),
+           (32, 32 | 64)]
 
 
 def dtype_of(prec):
