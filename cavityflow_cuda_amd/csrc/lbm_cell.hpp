@@ -250,32 +250,28 @@ __device__ __forceinline__ void alpha_cell(const StepArgs<typename P::C>& a, uin
     __builtin_amdgcn_sched_barrier(0);
     if (unpack) {
         uint32_t uov = 0;
+        // a cell that stores nothing (GHOST_LAYER, OBSTACLE) leaves the
+        // received values where the unpack would have put them: the unpack's
+        // codes, encoded from the buffer values (decoding and encoding again
+        // need not give them back); written here, so that the received values
+        // are not kept across the collision
+        const bool keep = flag == kGhostLayer || flag == kObstacle;
 #pragma unroll
         for (int k = 0; k < 5; k++) {
             const int e = xplane(k);
             const C v = xunpack_value<P>(u[k], e, uov);
             d[e] = odd ? d[e] : v;
             d[e + 1] = odd ? v : d[e + 1];
-        }
-        raise_overflow<P>(a.status, uov);
-    }
-    // a cell that stores nothing leaves the received values where the
-    // unpack would have put them: the unpack's codes, encoded from the
-    // buffer values (decoding and encoding again need not give them back)
-    auto keep_received = [&]() {
-        if (unpack) {
-#pragma unroll
-            for (int k = 0; k < 5; k++) {
-                const int e = xplane(k);
+            if (keep) {
                 if (odd)
                     acc.store(e + 1, 0, P::store(u[k], e));
                 else
                     acc.store(e, 0, P::store(u[k], e));
             }
         }
-    };
+        raise_overflow<P>(a.status, uov);
+    }
     if (flag == kGhostLayer) {   // packs its unchanged slots on a send plane
-        keep_received();
         xpack_alpha<P>(a, row, pl, pr, d);
         return;
     }
@@ -385,7 +381,6 @@ __device__ __forceinline__ void alpha_cell(const StepArgs<typename P::C>& a, uin
         }
     } else if (flag == kObstacle) {
         // no stores: the A-A layout makes bounce-back implicit (:330-343)
-        keep_received();
         if (pack)
             xpack_alpha<P>(a, row, pl, pr, d);
         if (a.store_u) {
@@ -440,14 +435,24 @@ __device__ __forceinline__ void beta_cell(const StepArgs<typename P::C>& a, uint
         xunpack_load<P, 1>(a, row, ul, odd, u);
     __builtin_amdgcn_sched_barrier(0);
     if (unpack) {
-        // slot l's value is d[opp(l)]: slot e in d[e + 1], slot e + 1 in d[e]
+        // slot l's value is d[opp(l)]: slot e in d[e + 1], slot e + 1 in d[e].
+        // OBSTACLE stores nothing: the received values stay where the unpack
+        // puts them (written here, so they are not kept across the collision)
         uint32_t uov = 0;
+        const bool keep = flag == kObstacle;
 #pragma unroll
         for (int k = 0; k < 5; k++) {
             const int e = xplane(k);
             const C v = xunpack_value<P>(u[k], e, uov);
             d[e + 1] = odd ? d[e + 1] : v;
             d[e] = odd ? v : d[e];
+            if (keep) {
+                if (odd)
+                    acc.store(e + 1, cx(e + 1) + cy(e + 1) * a.nx + cz(e + 1) * a.nxy,
+                              P::store(u[k], e));
+                else
+                    acc.store(e, cx(e) + cy(e) * a.nx + cz(e) * a.nxy, P::store(u[k], e));
+            }
         }
         raise_overflow<P>(a.status, uov);
     }
@@ -535,18 +540,6 @@ __device__ __forceinline__ void beta_cell(const StepArgs<typename P::C>& a, uint
         d[17] = eq_a1(uz, vela2, rho);
         d[18] = eq_18(rho);
     } else if (flag == kObstacle) {
-        if (unpack) {
-            // no stores: the received values stay where the unpack puts them
-#pragma unroll
-            for (int k = 0; k < 5; k++) {
-                const int e = xplane(k);
-                if (odd)
-                    acc.store(e + 1, cx(e + 1) + cy(e + 1) * a.nx + cz(e + 1) * a.nxy,
-                              P::store(u[k], e));
-                else
-                    acc.store(e, cx(e) + cy(e) * a.nx + cz(e) * a.nxy, P::store(u[k], e));
-            }
-        }
         if (pack) {
             // the swap writes back the value read from plane l (:302-311)
             C out[kQ];
