@@ -884,12 +884,18 @@ int lbm_exchange_face(lbm_solver* from, int direction, int phase, lbm_solver* to
     int rc = set_device(from);
     if (rc)
         return rc;
-    const size_t bytes = (size_t)lbm_face_elements(from, direction) * from->out_bytes;
-    if ((rc = ensure_staging(from, bytes)))
+    // lbm_pack_face(from, direction) followed by lbm_unpack_face(to, direction ^ 1)
+    // in one launch, with no buffer between them: the same boxes, planes and
+    // conversions
+    int so[3], sz[3], ro[3], rz[3];
+    if ((rc = lbm_face_box(from, direction, phase == 0 ? 0 : 1, so, sz)) ||
+        (rc = lbm_face_box(to, direction ^ 1, phase == 0 ? 1 : 0, ro, rz)))
         return rc;
-    if ((rc = lbm_pack_face(from, direction, phase, from->staging, stream)))
-        return rc;
-    return lbm_unpack_face(to, direction ^ 1, phase, from->staging, stream);
+    const Box sb{so[0], so[1], so[2], sz[0], sz[1], sz[2]};
+    const Box db{ro[0], ro[1], ro[2], rz[0], rz[1], rz[2]};
+    LBM_HIP(launch_dd_xfer(from, sb, to, db, exchange_mask(direction, phase), stream),
+            "exchange kernel");
+    return LBM_OK;
 }
 
 int lbm_device_alloc(int device, size_t bytes, void** out)

@@ -259,6 +259,57 @@ hipError_t launch_dd_set_pair(const lbm_solver* s, const Box* b, const void* con
     return copy_dd<false>(s, b, const_cast<void* const*>(src), mask, 2, st);
 }
 
+// One face exchange between two solvers of a device with no buffer between
+// them: plane l of `from`'s send box goes to the same cell of `to`'s
+// receive box (grid.y = index into the plane list), through the compute type
+// as a pack followed by an unpack takes it, the range of the stored value
+// marked on `to` as the unpack marks it. Both boxes have the same extents.
+template <class P>
+__global__ __launch_bounds__(kCopyBlock) void k_xfer_dd(const typename P::S* __restrict__ src,
+                                                        BoxArgs sa,
+                                                        typename P::S* __restrict__ dst,
+                                                        BoxArgs da)
+{
+    if ((int)blockIdx.y >= sa.nplanes)
+        return;
+    const int l = sa.plane[blockIdx.y];
+    const uint32_t gid = blockIdx.x * kCopyBlock + threadIdx.x;
+    if (gid >= sa.cells)
+        return;
+    const typename P::S v =
+        __builtin_nontemporal_load(src + (int64_t)l * sa.stride + box_index(sa, gid));
+    uint32_t ov = 0;
+    const typename P::S w = P::store_checked(P::load(v, l), l, ov);
+    raise_overflow<P>(da.status, ov);
+    __builtin_nontemporal_store(w, dst + (int64_t)l * da.stride + box_index(da, gid));
+}
+
+template <class P>
+static hipError_t xfer_dd_t(const lbm_solver* from, const Box& sb, lbm_solver* to,
+                            const Box& db, uint32_t mask, hipStream_t st)
+{
+    const BoxArgs sa = box_args(from, sb, mask), da = box_args(to, db, mask);
+    if (!sa.nplanes || !sa.cells)
+        return hipSuccess;
+    const dim3 grid((sa.cells + kCopyBlock - 1) / kCopyBlock, sa.nplanes);
+    hipLaunchKernelGGL((k_xfer_dd<P>), grid, dim3(kCopyBlock), 0, st,
+                       (const typename P::S*)from->dd, sa, (typename P::S*)to->dd, da);
+    return hipGetLastError();
+}
+
+hipError_t launch_dd_xfer(const lbm_solver* from, const Box& sb, lbm_solver* to, const Box& db,
+                          uint32_t mask, hipStream_t st)
+{
+    switch (from->p.precision) {
+    case LBM_F32: return xfer_dd_t<StoreF32>(from, sb, to, db, mask, st);
+    case LBM_F64: return xfer_dd_t<StoreF64>(from, sb, to, db, mask, st);
+    case LBM_F16S: return xfer_dd_t<StoreF16S>(from, sb, to, db, mask, st);
+    case LBM_BF16S: return xfer_dd_t<StoreBF16S>(from, sb, to, db, mask, st);
+    case LBM_F16C: return xfer_dd_t<StoreF16C>(from, sb, to, db, mask, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 template <class C>
 static hipError_t field_t(const lbm_solver* s, int field, const Box& b, void* buf, bool get,
                           hipStream_t st)

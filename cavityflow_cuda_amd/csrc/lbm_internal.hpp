@@ -114,6 +114,10 @@ hipError_t launch_box_get(const lbm_solver* s, int field, const Box& b, void* ds
 // each with its own buffer and plane mask
 hipError_t launch_dd_get_pair(const lbm_solver* s, const Box* b, void* const* dst,
                               const uint32_t* mask, hipStream_t st);
+// one face of `from` into `to` with no buffer (lbm_exchange_face); the boxes
+// have the same extents
+hipError_t launch_dd_xfer(const lbm_solver* from, const Box& sb, lbm_solver* to, const Box& db,
+                          uint32_t mask, hipStream_t st);
 hipError_t launch_dd_set_pair(const lbm_solver* s, const Box* b, const void* const* src,
                               const uint32_t* mask, hipStream_t st);
 // both x faces in one launch (nt: streaming stores)

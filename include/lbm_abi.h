@@ -357,11 +357,12 @@ int lbm_set_step_pack(lbm_solver* solver, void* left_buf, void* right_buf);
  * (CController.cpp:410-436, 848-874). */
 int lbm_set_step_unpack(lbm_solver* solver, const void* left_buf, const void* right_buf);
 
-/* Single-GPU stand-in for the neighbour send/recv of one face: packs `from`'s
- * face `direction` (phase as in lbm_pack_face) through `from`'s device
- * staging buffer and unpacks it into `to` across the opposite face, e.g.
- * from.RIGHT -> to.LEFT. Both solvers must live on the same device and have
- * the same halo-inclusive extent on the two axes of the face. */
+/* Single-GPU stand-in for the neighbour send/recv of one face: the result of
+ * lbm_pack_face(from, direction, phase) followed by lbm_unpack_face into `to`
+ * across the opposite face (e.g. from.RIGHT -> to.LEFT), in one launch that
+ * copies the 5 populations from `from`'s face straight into `to`'s (no
+ * buffer). Both solvers must live on the same device and have the same
+ * halo-inclusive extent on the two axes of the face. */
 int lbm_exchange_face(lbm_solver* from, int direction, int phase, lbm_solver* to,
                       hipStream_t stream);
 
