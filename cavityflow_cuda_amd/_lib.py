@@ -33,7 +33,7 @@ EXPORTED = (
     "lbm_get_velocities", "lbm_set_velocities", "lbm_get_densities", "lbm_set_densities",
     "lbm_face_box", "lbm_face_elements", "lbm_pack_face", "lbm_unpack_face",
     "lbm_unpack_face_pair", "lbm_pack_face_pair",
-    "lbm_exchange_face", "lbm_device_alloc", "lbm_device_free",
+    "lbm_exchange_face", "lbm_exchange_faces", "lbm_device_alloc", "lbm_device_free",
     "lbm_set_launch_options", "lbm_get_launch_options", "lbm_set_step_pack",
     "lbm_set_step_unpack", "lbm_get_moments",
     "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_size", "lbm_batch_set_tuning",
@@ -103,6 +103,14 @@ class lbm_batch_tuning(ctypes.Structure):
     ]
 
 
+class lbm_face_transfer(ctypes.Structure):
+    _fields_ = [
+        ("from_", ctypes.c_void_p),
+        ("direction", ctypes.c_int),
+        ("to", ctypes.c_void_p),
+    ]
+
+
 class lbm_device_arrays(ctypes.Structure):
     _fields_ = [
         ("dd", ctypes.c_void_p),
@@ -159,6 +167,7 @@ def lib() -> ctypes.CDLL:
         "lbm_unpack_face_pair": ([P, I, I, P, P, P], I),
         "lbm_pack_face_pair": ([P, I, I, P, P, P], I),
         "lbm_exchange_face": ([P, I, I, P, P], I),
+        "lbm_exchange_faces": ([P, I, I, P], I),
         "lbm_device_alloc": ([I, ctypes.c_size_t, ctypes.POINTER(P)], I),
         "lbm_device_free": ([I, P], I),
         "lbm_set_launch_options": ([P, I, I], I),

@@ -14,7 +14,9 @@ surface of the reference, on one GPU.
   (CController.cpp:1278-1287). After each half-step the subdomains exchange
   faces on the device in the reference order x, then y, then z
   (CController.cpp:369-570 after alpha, 807-1008 after beta), with
-  ``lbm_exchange_face`` standing in for the MPI send/recv pair.
+  ``lbm_exchange_face`` standing in for the MPI send/recv pair; the transfers
+  of one axis are independent, so ``lbm_exchange_faces`` runs each axis's in
+  one launch.
 * ``overlap=True`` keeps the reference's step order for that exchange
   (CController.cpp:209-309, 647-747): each subdomain first steps the two
   cells next to every communicating face (the halo and send planes), then its
@@ -104,6 +106,20 @@ class CavityController:
             if j == py - 1:
                 s.setDrivenCavitySzenario()
             self.solvers[(i, j, k)] = s
+        # the face transfers of each axis (both ways between every pair of
+        # neighbours), handed to one batched exchange per axis
+        self._transfers = []
+        for axis in range(3):
+            t = []
+            for key, a in self.solvers.items():
+                nb = list(key)
+                nb[axis] += 1
+                b = self.solvers.get(tuple(nb))
+                if b is not None:
+                    t.append((a.handle.value, 2 * axis + 1, b.handle.value))
+                    t.append((b.handle.value, 2 * axis, a.handle.value))
+            if t:
+                self._transfers.append(((_lib.lbm_face_transfer * len(t))(*t), len(t)))
         self.overlap = bool(overlap) and len(self.solvers) > 1
         # x slabs may be thicker than the reference's two cells: a slab two cells
         # wide touches one cache line per row and plane for 8 bytes of data
@@ -279,16 +295,8 @@ class CavityController:
     def _exchange(self, phase: int, stream=None):
         L = lib()
         st = _stream(self.stream if stream is None else stream)
-        for axis in range(3):
-            lo_dir, hi_dir = 2 * axis, 2 * axis + 1
-            for key, a in self.solvers.items():
-                nb = list(key)
-                nb[axis] += 1
-                b = self.solvers.get(tuple(nb))
-                if b is None:
-                    continue
-                check(L.lbm_exchange_face(a.handle, hi_dir, phase, b.handle, st))
-                check(L.lbm_exchange_face(b.handle, lo_dir, phase, a.handle, st))
+        for arr, n in self._transfers:   # x, then y, then z
+            check(L.lbm_exchange_faces(arr, n, phase, st))
 
     def stepAlpha(self):
         if self.overlap:

@@ -181,9 +181,12 @@ public:
         return solvers_[(k * o_.split[1] + j) * o_.split[0] + i].get();
     }
 
+    // the faces in the reference order x, then y, then z; the transfers of
+    // one axis are independent and go to lbm_exchange_faces together
     void exchange(int phase)
     {
-        for (int axis = 0; axis < 3; axis++)
+        for (int axis = 0; axis < 3; axis++) {
+            std::vector<lbm_face_transfer> t;
             for (int k = 0; k < o_.split[2]; k++)
                 for (int j = 0; j < o_.split[1]; j++)
                     for (int i = 0; i < o_.split[0]; i++) {
@@ -193,11 +196,11 @@ public:
                             continue;
                         auto* a = at(i, j, k);
                         auto* b = at(nb[0], nb[1], nb[2]);
-                        lbm::check(lbm_exchange_face(a->handle(), 2 * axis + 1, phase, b->handle(),
-                                                     nullptr));
-                        lbm::check(lbm_exchange_face(b->handle(), 2 * axis, phase, a->handle(),
-                                                     nullptr));
+                        t.push_back({a->handle(), 2 * axis + 1, b->handle()});
+                        t.push_back({b->handle(), 2 * axis, a->handle()});
                     }
+            lbm::check(lbm_exchange_faces(t.data(), (int)t.size(), phase, nullptr));
+        }
     }
 
     // x faces exchanged inside the step kernels: every subdomain packs its

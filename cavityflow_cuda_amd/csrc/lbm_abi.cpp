@@ -9,6 +9,7 @@
 #include <cstring>
 #include <limits>
 #include <string>
+#include <vector>
 
 #include "lbm_abi.h"
 #include "lbm_d3q19.hpp"
@@ -864,37 +865,82 @@ int lbm_set_step_unpack(lbm_solver* s, const void* left_buf, const void* right_b
     return LBM_OK;
 }
 
-int lbm_exchange_face(lbm_solver* from, int direction, int phase, lbm_solver* to,
-                      hipStream_t stream)
+// lbm_pack_face(from, direction) followed by lbm_unpack_face(to, direction ^ 1)
+// as one buffer-less transfer: the same boxes, planes and conversions
+static int exchange_desc(const char* fn, lbm_solver* from, int direction, int phase,
+                         lbm_solver* to, XferDesc* d)
 {
     if (!from || !to || direction < 0 || direction > 5 || phase < 0 || phase > 1) {
-        set_error("lbm_exchange_face: invalid argument");
+        set_error(std::string(fn) + ": invalid argument");
         return LBM_ERR_ARG;
     }
     if (from->p.device != to->p.device || from->p.precision != to->p.precision) {
-        set_error("lbm_exchange_face: solvers differ in device or precision");
+        set_error(std::string(fn) + ": solvers differ in device or precision");
         return LBM_ERR_ARG;
     }
     const int axis = direction / 2;
     for (int i = 0; i < 3; i++)
         if (i != axis && from->size_h[i] != to->size_h[i]) {
-            set_error("lbm_exchange_face: face extents differ");
+            set_error(std::string(fn) + ": face extents differ");
             return LBM_ERR_ARG;
         }
-    int rc = set_device(from);
-    if (rc)
-        return rc;
-    // lbm_pack_face(from, direction) followed by lbm_unpack_face(to, direction ^ 1)
-    // in one launch, with no buffer between them: the same boxes, planes and
-    // conversions
     int so[3], sz[3], ro[3], rz[3];
+    int rc;
     if ((rc = lbm_face_box(from, direction, phase == 0 ? 0 : 1, so, sz)) ||
         (rc = lbm_face_box(to, direction ^ 1, phase == 0 ? 1 : 0, ro, rz)))
         return rc;
-    const Box sb{so[0], so[1], so[2], sz[0], sz[1], sz[2]};
-    const Box db{ro[0], ro[1], ro[2], rz[0], rz[1], rz[2]};
-    LBM_HIP(launch_dd_xfer(from, sb, to, db, exchange_mask(direction, phase), stream),
-            "exchange kernel");
+    d->from = from;
+    d->sb = Box{so[0], so[1], so[2], sz[0], sz[1], sz[2]};
+    d->to = to;
+    d->db = Box{ro[0], ro[1], ro[2], rz[0], rz[1], rz[2]};
+    d->mask = exchange_mask(direction, phase);
+    return LBM_OK;
+}
+
+int lbm_exchange_face(lbm_solver* from, int direction, int phase, lbm_solver* to,
+                      hipStream_t stream)
+{
+    XferDesc d;
+    int rc = exchange_desc("lbm_exchange_face", from, direction, phase, to, &d);
+    if (rc || (rc = set_device(from)))
+        return rc;
+    LBM_HIP(launch_dd_xfer(&d, 1, stream), "exchange kernel");
+    return LBM_OK;
+}
+
+int lbm_exchange_faces(const lbm_face_transfer* transfers, int n, int phase,
+                       hipStream_t stream)
+{
+    if (n < 0 || (n > 0 && !transfers)) {
+        set_error("lbm_exchange_faces: invalid argument");
+        return LBM_ERR_ARG;
+    }
+    if (n == 0)
+        return LBM_OK;
+    std::vector<XferDesc> list((size_t)n);
+    for (int k = 0; k < n; k++) {
+        const lbm_face_transfer& t = transfers[k];
+        int rc = exchange_desc("lbm_exchange_faces", t.from, t.direction, phase, t.to, &list[k]);
+        if (rc)
+            return rc;
+        const lbm_face_transfer& t0 = transfers[0];
+        if (t.direction / 2 != t0.direction / 2 || t.from->p.device != t0.from->p.device ||
+            t.from->p.precision != t0.from->p.precision) {
+            set_error("lbm_exchange_faces: transfers differ in axis, device or precision");
+            return LBM_ERR_ARG;
+        }
+        // one axis: sends read the planes one side of the faces, receives
+        // write the others, so only two transfers into the same face collide
+        for (int j = 0; j < k; j++)
+            if (transfers[j].to == t.to && transfers[j].direction == t.direction) {
+                set_error("lbm_exchange_faces: two transfers write the same face");
+                return LBM_ERR_ARG;
+            }
+    }
+    int rc = set_device(transfers[0].from);
+    if (rc)
+        return rc;
+    LBM_HIP(launch_dd_xfer(list.data(), n, stream), "exchange kernel");
     return LBM_OK;
 }
 

@@ -259,23 +259,37 @@ hipError_t launch_dd_set_pair(const lbm_solver* s, const Box* b, const void* con
     return copy_dd<false>(s, b, const_cast<void* const*>(src), mask, 2, st);
 }
 
-// One face exchange between two solvers of a device with no buffer between
-// them: plane l of `from`'s send box goes to the same cell of `to`'s
-// receive box (grid.y = index into the plane list), through the compute type
-// as a pack followed by an unpack takes it, the range of the stored value
-// marked on `to` as the unpack marks it. Both boxes have the same extents.
+// Face exchanges between solvers of a device with no buffer between them,
+// up to kXferBatch in one launch: plane l of a transfer's send box goes to the
+// same cell of its receive box (grid.y = index into the plane list), through
+// the compute type as a pack followed by an unpack takes it, the range of the
+// stored value marked on the receiver as the unpack marks it. The two boxes
+// of a transfer have the same extents; grid.x is the transfers' blocks one
+// after the other, and the transfers of one launch touch disjoint elements.
+struct XferBatch {
+    const void* src[kXferBatch];
+    void* dst[kXferBatch];
+    BoxArgs sa[kXferBatch], da[kXferBatch];
+    uint32_t block0[kXferBatch + 1];   // first block of each transfer, then the total
+    int n;
+};
+
 template <class P>
-__global__ __launch_bounds__(kCopyBlock) void k_xfer_dd(const typename P::S* __restrict__ src,
-                                                        BoxArgs sa,
-                                                        typename P::S* __restrict__ dst,
-                                                        BoxArgs da)
+__global__ __launch_bounds__(kCopyBlock) void k_xfer_dd(const XferBatch b)
 {
+    int t = 0;
+    while (t + 1 < b.n && blockIdx.x >= b.block0[t + 1])
+        t++;
+    const BoxArgs& sa = b.sa[t];
+    const BoxArgs& da = b.da[t];
     if ((int)blockIdx.y >= sa.nplanes)
         return;
     const int l = sa.plane[blockIdx.y];
-    const uint32_t gid = blockIdx.x * kCopyBlock + threadIdx.x;
+    const uint32_t gid = (blockIdx.x - b.block0[t]) * kCopyBlock + threadIdx.x;
     if (gid >= sa.cells)
         return;
+    const typename P::S* src = (const typename P::S*)b.src[t];
+    typename P::S* dst = (typename P::S*)b.dst[t];
     const typename P::S v =
         __builtin_nontemporal_load(src + (int64_t)l * sa.stride + box_index(sa, gid));
     uint32_t ov = 0;
@@ -285,27 +299,47 @@ __global__ __launch_bounds__(kCopyBlock) void k_xfer_dd(const typename P::S* __r
 }
 
 template <class P>
-static hipError_t xfer_dd_t(const lbm_solver* from, const Box& sb, lbm_solver* to,
-                            const Box& db, uint32_t mask, hipStream_t st)
+static hipError_t xfer_dd_t(const XferDesc* list, int n, hipStream_t st)
 {
-    const BoxArgs sa = box_args(from, sb, mask), da = box_args(to, db, mask);
-    if (!sa.nplanes || !sa.cells)
-        return hipSuccess;
-    const dim3 grid((sa.cells + kCopyBlock - 1) / kCopyBlock, sa.nplanes);
-    hipLaunchKernelGGL((k_xfer_dd<P>), grid, dim3(kCopyBlock), 0, st,
-                       (const typename P::S*)from->dd, sa, (typename P::S*)to->dd, da);
-    return hipGetLastError();
+    for (int first = 0; first < n; first += kXferBatch) {
+        XferBatch b{};
+        int nplanes = 0;
+        uint32_t blocks = 0;
+        for (int k = first; k < n && k < first + kXferBatch; k++) {
+            const XferDesc& d = list[k];
+            const BoxArgs sa = box_args(d.from, d.sb, d.mask);
+            if (!sa.nplanes || !sa.cells)
+                continue;
+            b.src[b.n] = d.from->dd;
+            b.dst[b.n] = d.to->dd;
+            b.sa[b.n] = sa;
+            b.da[b.n] = box_args(d.to, d.db, d.mask);
+            b.block0[b.n] = blocks;
+            blocks += (sa.cells + kCopyBlock - 1) / kCopyBlock;
+            nplanes = sa.nplanes > nplanes ? sa.nplanes : nplanes;
+            b.n++;
+        }
+        if (!b.n)
+            continue;
+        b.block0[b.n] = blocks;
+        hipLaunchKernelGGL((k_xfer_dd<P>), dim3(blocks, nplanes), dim3(kCopyBlock), 0, st, b);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess)
+            return e;
+    }
+    return hipSuccess;
 }
 
-hipError_t launch_dd_xfer(const lbm_solver* from, const Box& sb, lbm_solver* to, const Box& db,
-                          uint32_t mask, hipStream_t st)
+hipError_t launch_dd_xfer(const XferDesc* list, int n, hipStream_t st)
 {
-    switch (from->p.precision) {
-    case LBM_F32: return xfer_dd_t<StoreF32>(from, sb, to, db, mask, st);
-    case LBM_F64: return xfer_dd_t<StoreF64>(from, sb, to, db, mask, st);
-    case LBM_F16S: return xfer_dd_t<StoreF16S>(from, sb, to, db, mask, st);
-    case LBM_BF16S: return xfer_dd_t<StoreBF16S>(from, sb, to, db, mask, st);
-    case LBM_F16C: return xfer_dd_t<StoreF16C>(from, sb, to, db, mask, st);
+    if (n <= 0)
+        return hipSuccess;
+    switch (list[0].from->p.precision) {
+    case LBM_F32: return xfer_dd_t<StoreF32>(list, n, st);
+    case LBM_F64: return xfer_dd_t<StoreF64>(list, n, st);
+    case LBM_F16S: return xfer_dd_t<StoreF16S>(list, n, st);
+    case LBM_BF16S: return xfer_dd_t<StoreBF16S>(list, n, st);
+    case LBM_F16C: return xfer_dd_t<StoreF16C>(list, n, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -114,10 +114,19 @@ hipError_t launch_box_get(const lbm_solver* s, int field, const Box& b, void* ds
 // each with its own buffer and plane mask
 hipError_t launch_dd_get_pair(const lbm_solver* s, const Box* b, void* const* dst,
                               const uint32_t* mask, hipStream_t st);
-// one face of `from` into `to` with no buffer (lbm_exchange_face); the boxes
-// have the same extents
-hipError_t launch_dd_xfer(const lbm_solver* from, const Box& sb, lbm_solver* to, const Box& db,
-                          uint32_t mask, hipStream_t st);
+// faces of one solver copied into another's with no buffer
+// (lbm_exchange_face[s]): the boxes of a transfer have the same extents, the
+// solvers one device and precision, and no element is written by one
+// transfer of the list and read or written by another
+struct XferDesc {
+    const lbm_solver* from;
+    Box sb;
+    lbm_solver* to;
+    Box db;
+    uint32_t mask;
+};
+constexpr int kXferBatch = 8;   // transfers per launch
+hipError_t launch_dd_xfer(const XferDesc* list, int n, hipStream_t st);
 hipError_t launch_dd_set_pair(const lbm_solver* s, const Box* b, const void* const* src,
                               const uint32_t* mask, hipStream_t st);
 // both x faces in one launch (nt: streaming stores)

@@ -366,6 +366,29 @@ int lbm_set_step_unpack(lbm_solver* solver, const void* left_buf, const void* ri
 int lbm_exchange_face(lbm_solver* from, int direction, int phase, lbm_solver* to,
                       hipStream_t stream);
 
+/* One face transfer of lbm_exchange_faces: from's `direction` face into
+ * to's opposite face. */
+typedef struct lbm_face_transfer {
+    lbm_solver* from;
+    int direction;
+    lbm_solver* to;
+} lbm_face_transfer;
+
+/* lbm_exchange_face for each of `n` transfers of one phase, with the same
+ * result as those calls one after another, in ceil(n / 8) launches: all the
+ * transfers of a decomposition along one axis (the faces between every pair
+ * of neighbours on that axis, both ways). All transfers share one axis, device
+ * and precision, and no two write the same face (same `to` and `direction`);
+ * otherwise LBM_ERR_ARG and nothing runs. On one axis a phase's sends read
+ * the face planes on one side of every face (phase 0 the last interior plane,
+ * phase 1 the halo) and its receives write the planes on the other, so the
+ * transfers are independent. The axes still go in order x, y, z (the y and z
+ * faces carry x-halo cells). No reference counterpart: the reference
+ * exchanges each face with its own MPI send/recv pair (CController.cpp:369-570,
+ * 807-1008). */
+int lbm_exchange_faces(const lbm_face_transfer* transfers, int n, int phase,
+                       hipStream_t stream);
+
 /* Device memory for face buffers and other caller-owned scratch. */
 int lbm_device_alloc(int device, size_t bytes, void** out);
 int lbm_device_free(int device, void* ptr);

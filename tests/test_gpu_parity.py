@@ -214,6 +214,97 @@ def test_fused_x_exchange_rejects_other_splits(hip):
             hip.CavityController((8, 8, 8), fused_x=True, **kw)
 
 
+def _transfers(pairs):
+    from cavityflow_cuda_amd import _lib
+    arr = (_lib.lbm_face_transfer * len(pairs))()
+    for k, (a, d, b) in enumerate(pairs):
+        arr[k].from_, arr[k].direction, arr[k].to = a.handle.value, d, b.handle.value
+    return arr
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64", "f16s", "bf16s", "f16c"])
+def test_exchange_faces_equals_single_exchanges(hip, prec):
+    """lbm_exchange_faces over all the transfers of one axis (3 x 3 subdomains
+    in chains of 3 along it: 12 transfers, two launches) leaves every solver
+    exactly as lbm_exchange_face on each transfer in turn, for every axis and
+    phase; so does a solver exchanging its two faces of an axis with itself
+    (a wall reflected through the exchange)."""
+    from cavityflow_cuda_amd._lib import check, lib
+    say("exchange_faces", prec)
+    rng = np.random.default_rng(53)
+    dt = np.float64 if prec == "f64" else np.float32
+    w = np.array([1 / 18] * 4 + [1 / 36] * 12 + [1 / 18] * 2 + [1 / 3])
+    size = (5, 4, 3)
+    cells = int(np.prod([v + 2 for v in size]))
+    L = lib()
+    for axis in range(3):
+        for phase in (0, 1):
+            grids = []
+            starts = [(w[:, None] * (1 + 0.1 * rng.standard_normal((19, cells)))).astype(dt).ravel()
+                      for _ in range(9)]
+            for _ in range(2):
+                g = [hip.LbmSolverHIP(size, (GHOST,) * 6, prec) for _ in range(9)]
+                for sv, st in zip(g, starts):
+                    sv.setDensityDistributions(st, *full_box(sv))
+                grids.append(g)
+            pairs = []
+            for c in range(3):              # chains of 3 along the axis
+                for i in range(2):
+                    a, b = c * 3 + i, c * 3 + i + 1
+                    pairs.append((a, 2 * axis + 1, b))
+                    pairs.append((b, 2 * axis, a))
+            batched, single = grids
+            check(L.lbm_exchange_faces(_transfers([(batched[a], d, batched[b]) for a, d, b in pairs]),
+                                       len(pairs), phase, None))
+            for a, d, b in pairs:
+                check(L.lbm_exchange_face(single[a].handle, d, phase, single[b].handle, None))
+            # self exchange of solver 0's two faces on the axis
+            self_pairs = [(0, 2 * axis + 1, 0), (0, 2 * axis, 0)]
+            check(L.lbm_exchange_faces(_transfers([(batched[a], d, batched[b])
+                                                   for a, d, b in self_pairs]), 2, phase, None))
+            for a, d, b in self_pairs:
+                check(L.lbm_exchange_face(single[a].handle, d, phase, single[b].handle, None))
+            for x, y in zip(batched, single):
+                x.sync()
+                y.sync()
+                assert x.getDensityDistributions(*full_box(x)).tobytes() == \
+                    y.getDensityDistributions(*full_box(y)).tobytes(), (axis, phase)
+                fx = bool(np.isnan(x.getMoments(0, (1, 1, 1), (1, 1, 1))).any())
+                fy = bool(np.isnan(y.getMoments(0, (1, 1, 1), (1, 1, 1))).any())
+                assert fx == fy
+            for g in grids:
+                for sv in g:
+                    sv.close()
+
+
+def test_exchange_faces_rejections(hip):
+    """Transfers on two axes, two into the same face, solvers of another
+    precision, a negative count or a NULL list are argument errors and run
+    nothing; an empty list is a no-op."""
+    from cavityflow_cuda_amd import _lib
+    from cavityflow_cuda_amd._lib import lib
+    say("exchange_faces rejections")
+    L = lib()
+    a = hip.LbmSolverHIP((4, 4, 4), (GHOST,) * 6, "f32")
+    b = hip.LbmSolverHIP((4, 4, 4), (GHOST,) * 6, "f32")
+    c = hip.LbmSolverHIP((4, 4, 4), (GHOST,) * 6, "f16s")
+    before = [s.getDensityDistributions(*full_box(s)).tobytes() for s in (a, b)]
+    for pairs in ([(a, 1, b), (b, 2, a)],          # x and y
+                  [(a, 1, b), (a, 1, b)],          # b's LEFT face twice
+                  [(a, 1, b), (c, 0, a)]):         # another precision
+        assert L.lbm_exchange_faces(_transfers(pairs), len(pairs), 0, None) == _lib.LBM_ERR_ARG
+    ok = _transfers([(a, 1, b)])
+    assert L.lbm_exchange_faces(ok, 1, 2, None) == _lib.LBM_ERR_ARG
+    assert L.lbm_exchange_faces(ok, -1, 0, None) == _lib.LBM_ERR_ARG
+    assert L.lbm_exchange_faces(None, 1, 0, None) == _lib.LBM_ERR_ARG
+    assert L.lbm_exchange_faces(None, 0, 0, None) == _lib.LBM_OK
+    for s, want in zip((a, b), before):
+        s.sync()
+        assert s.getDensityDistributions(*full_box(s)).tobytes() == want
+    for s in (a, b, c):
+        s.close()
+
+
 def test_overlap_rejects_subdomains_too_thin(hip):
     with pytest.raises(ValueError):
         hip.CavityController((2, 8, 8), split=(2, 1, 1), overlap=True)
