@@ -833,7 +833,7 @@ static void refresh_launch_defaults(lbm_solver* s)
     for (int k = 0; k < 2; k++)
         if (!s->launch_set[k])
             s->launch[k] = default_launch(s->p.precision, k, with_pack, s->n >= kWideCells,
-                                          with_unpack);
+                                          with_unpack, !plane_off32(s));
 }
 
 int lbm_set_step_pack(lbm_solver* s, void* left_buf, void* right_buf)

@@ -185,8 +185,18 @@ hipError_t launch_reduce_boxes(const lbm_solver* const* s, const Box* b, int cou
 // (F16C beta, with more integer work per cell, loses with them).
 constexpr int64_t kWideCells = 6 << 20;   // halo-inclusive cells for `wide`
 
+// 32-bit byte offsets reach every element of a plane
+inline bool plane_off32(const lbm_solver* s)
+{
+    return (uint64_t)s->stride * (uint64_t)s->elem_bytes < (1ull << 32);
+}
+
+// addr64: the planes need 64-bit element offsets (4 GiB and up); the FP32
+// beta with the fused unpack in 1024-thread workgroups then runs out of its
+// 128 registers per thread and spills (half the step rate at 512^3 forced to
+// 64-bit offsets), so it keeps 512-thread workgroups.
 inline int default_launch(int precision, int kernel, bool with_pack, bool wide,
-                          bool with_unpack = false)
+                          bool with_unpack = false, bool addr64 = false)
 {
     switch (precision) {
     case LBM_F32:
@@ -198,6 +208,8 @@ inline int default_launch(int precision, int kernel, bool with_pack, bool wide,
                 return LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_WAVES3;
             return (wide ? LBM_LAUNCH_WIDE : 0) | LBM_LAUNCH_NONTEMPORAL | LBM_LAUNCH_WAVES2;
         }
+        if (with_unpack && wide && addr64)
+            return LBM_LAUNCH_WIDE | LBM_LAUNCH_WAVES3;
         if ((with_pack || with_unpack) && wide)
             return LBM_LAUNCH_BIG | LBM_LAUNCH_WAVES4;
         return (wide ? LBM_LAUNCH_WIDE : 0) | LBM_LAUNCH_WAVES2;

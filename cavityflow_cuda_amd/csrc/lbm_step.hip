@@ -315,11 +315,6 @@ static hipError_t init_t(const lbm_solver* s, hipStream_t st, int write_flags)
     return hipGetLastError();
 }
 
-// 32-bit byte offsets reach every element of a plane
-static bool off32(const lbm_solver* s)
-{
-    return (uint64_t)s->stride * (uint64_t)s->elem_bytes < (1ull << 32);
-}
 
 // LBM_LAUNCH_WAVES4 / WAVES3 / WAVES2: reserving this much (unused) LDS per
 // 256-thread workgroup leaves room for 4 (3, 2) workgroups in a CU's 160 KB,
@@ -353,7 +348,7 @@ static void launch_shape(const lbm_solver* s, const StepArgs<typename P::C>& a, 
                        : (opts & LBM_LAUNCH_WAVES3) ? kWaves3Lds
                        : (opts & LBM_LAUNCH_WAVES4) ? kWaves4Lds : 0;
     const dim3 grid(blocks), block(G::kThreads);
-    if (!off32(s) || (opts & LBM_LAUNCH_ADDR64))
+    if (!plane_off32(s) || (opts & LBM_LAUNCH_ADDR64))
         hipLaunchKernelGGL((K<P, FULL, false, false, SHAPE, XF>::kernel), grid, block, lds, st, a);
     else if (opts & LBM_LAUNCH_NONTEMPORAL)
         hipLaunchKernelGGL((K<P, FULL, true, true, SHAPE, XF>::kernel), grid, block, lds, st, a);
@@ -452,7 +447,7 @@ static hipError_t beta_t(const lbm_solver* s, const Box& b, hipStream_t st)
         a.plane[l] += (int64_t)(cx(l) + cy(l) * a.nx + cz(l) * a.nxy) *
                       (int64_t)sizeof(typename P::S);
     const int opts = s->launch[1];
-    if ((opts & LBM_LAUNCH_LDS) && is_full(s, b) && off32(s) && !(opts & LBM_LAUNCH_ADDR64)) {
+    if ((opts & LBM_LAUNCH_LDS) && is_full(s, b) && plane_off32(s) && !(opts & LBM_LAUNCH_ADDR64)) {
         launch_beta_lds<P>(s, a, opts, st);
         return hipGetLastError();
     }

@@ -176,7 +176,8 @@ int lbm_simulation_step_beta(lbm_solver* solver, const int origin[3], const int 
  * kernel: 0 = alpha, 1 = beta. Defaults (measured on MI355X): F32 beta
  * WIDE | WAVES2 on domains of 6 Mi cells or more (halo included: from 183^3
  * interior cells), BIG | WAVES4 there with a step pack or unpack
- * (lbm_set_step_pack / lbm_set_step_unpack), and WAVES2 below; F32 alpha
+ * (lbm_set_step_pack / lbm_set_step_unpack) except WIDE | WAVES3 with an
+ * unpack on domains whose planes exceed 4 GiB, and WAVES2 below; F32 alpha
  * NONTEMPORAL | WAVES2 plus WIDE on those large domains, NONTEMPORAL | WAVES3
  * with a step unpack, or with a step pack on smaller domains; F64 alpha
  * NONTEMPORAL | WAVES2, beta LDS | WAVES2; F16S / BF16S alpha none, beta BIG
@@ -206,8 +207,10 @@ enum lbm_launch_option {
     LBM_LAUNCH_BIG = 256         /* 1024-thread workgroups, one cell per thread
                                     (wins over WIDE; a workgroup alone fills 4
                                     waves per SIMD, so any WAVES cap holds one per
-                                    CU; single-domain kernels only; F64 beta spills
-                                    registers in this shape) */
+                                    CU; single-domain kernels only; beta spills
+                                    registers in this shape for F64, and with a
+                                    step unpack and 64-bit indexing for the
+                                    other formats) */
 };
 int lbm_set_launch_options(lbm_solver* solver, int kernel, int options);
 int lbm_get_launch_options(const lbm_solver* solver, int kernel, int* options);

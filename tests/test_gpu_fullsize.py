@@ -78,7 +78,8 @@ def test_64bit_indexing_domain_equals_split(hip):
     """A 1024^3 cavity (1026^3 cells: each population plane 4.32 GB, beyond
     32-bit byte offsets, so the step kernels take their 64-bit indexing path)
     against the same cavity split 2x1x1 (planes of 2.16 GB: the 32-bit path),
-    bit for bit after 8 steps with the on-device face exchange."""
+    bit for bit after 8 steps with the on-device face exchange; and against
+    itself with its x faces exchanged inside the step kernels."""
     n, steps = 1024, 8
     say("64-bit indexing", n, steps)
     c = hip.CavityController((n, n, n), u_lid=0.1, reynolds=1000.0, precision="f32",
@@ -102,3 +103,39 @@ def test_64bit_indexing_domain_equals_split(hip):
     s.close()
     assert np.abs(u_mono).max() > 0.05
     assert u.tobytes() == u_mono.tobytes()
+    # the same cavity receiving its own x faces through the fused pack and
+    # unpack (two buffer sets swapping every alpha + beta pair): with 64-bit
+    # indexing FP32 beta then defaults to 512-thread workgroups, the
+    # 1024-thread kernel would spill
+    import ctypes
+    from hiprt import DeviceBuffer
+    from cavityflow_cuda_amd._lib import (LAUNCH_NONTEMPORAL, LAUNCH_WAVES3, LAUNCH_WIDE, check,
+                                          lib)
+    say("64-bit indexing, self-exchanged x faces", n, steps)
+    f = hip.CavityController((n, n, n), u_lid=0.1, reynolds=1000.0, precision="f32",
+                             store_outputs=True)
+    sv = next(iter(f.solvers.values()))
+    ne = sv.face_elements(0)
+    bufs = [DeviceBuffer(ne * 4) for _ in range(4)]
+    sets = [bufs[:2], bufs[2:]]
+    for side in (0, 1):
+        sv.pack_face(side, 0, sets[1][side])
+        sv.pack_face(side, 1, sets[1][side])
+    v = ctypes.c_int()
+    for step in range(steps):
+        j = (step >> 1) & 1
+        sv.set_step_pack(*sets[j])
+        sv.set_step_unpack(*sets[1 - j])
+        if step == 0:
+            for k, want in ((0, LAUNCH_NONTEMPORAL | LAUNCH_WAVES3), (1, LAUNCH_WIDE | LAUNCH_WAVES3)):
+                check(lib().lbm_get_launch_options(sv.handle, k, ctypes.byref(v)))
+                assert v.value == want, k
+        f.computeNextStep()
+    f.sync()
+    u_fused = f.getVelocities()
+    sv.set_step_pack(None, None)
+    sv.set_step_unpack(None, None)
+    f.close()
+    for b in bufs:
+        b.free()
+    assert u_fused.tobytes() == u_mono.tobytes()
