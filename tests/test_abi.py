@@ -63,6 +63,22 @@ def test_create_rejects_bad_arguments_without_a_gpu(hip):
     assert L.lbm_status_string(-2) == b"box outside the domain"
 
 
+def test_face_exchanges_reject_bad_arguments_without_a_gpu(hip):
+    """The buffer-less face exchanges check their arguments before any device
+    work: NULL solvers, a negative count or a NULL list are argument errors,
+    an empty list is a no-op."""
+    from cavityflow_cuda_amd import _lib
+    L = _lib.lib()
+    assert L.lbm_exchange_face(None, 1, 0, None, None) == _lib.LBM_ERR_ARG
+    arr = (_lib.lbm_face_transfer * 2)()
+    arr[0].direction = arr[1].direction = 1
+    assert L.lbm_exchange_faces(arr, 2, 0, None) == _lib.LBM_ERR_ARG
+    assert b"invalid argument" in L.lbm_last_error()
+    assert L.lbm_exchange_faces(arr, -1, 0, None) == _lib.LBM_ERR_ARG
+    assert L.lbm_exchange_faces(None, 3, 1, None) == _lib.LBM_ERR_ARG
+    assert L.lbm_exchange_faces(None, 0, 1, None) == _lib.LBM_OK
+
+
 def test_compute_parameters_match_reference_golden(hip):
     """CLbmSolver constructor adaptation: bit-exact against the reference (host code)."""
     from cavityflow_cuda_amd import LbmError, compute_parameters
