@@ -3,12 +3,14 @@
 // come back as lbm_status codes with a per-thread message.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <limits>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "lbm_abi.h"
@@ -918,6 +920,7 @@ int lbm_exchange_faces(const lbm_face_transfer* transfers, int n, int phase,
     if (n == 0)
         return LBM_OK;
     std::vector<XferDesc> list((size_t)n);
+    std::vector<std::pair<uintptr_t, int>> written((size_t)n);
     for (int k = 0; k < n; k++) {
         const lbm_face_transfer& t = transfers[k];
         int rc = exchange_desc("lbm_exchange_faces", t.from, t.direction, phase, t.to, &list[k]);
@@ -929,13 +932,14 @@ int lbm_exchange_faces(const lbm_face_transfer* transfers, int n, int phase,
             set_error("lbm_exchange_faces: transfers differ in axis, device or precision");
             return LBM_ERR_ARG;
         }
-        // one axis: sends read the planes one side of the faces, receives
-        // write the others, so only two transfers into the same face collide
-        for (int j = 0; j < k; j++)
-            if (transfers[j].to == t.to && transfers[j].direction == t.direction) {
-                set_error("lbm_exchange_faces: two transfers write the same face");
-                return LBM_ERR_ARG;
-            }
+        written[k] = {(uintptr_t)t.to, t.direction};
+    }
+    // one axis: sends read the planes one side of the faces, receives write
+    // the others, so only two transfers into the same face collide
+    std::sort(written.begin(), written.end());
+    if (std::adjacent_find(written.begin(), written.end()) != written.end()) {
+        set_error("lbm_exchange_faces: two transfers write the same face");
+        return LBM_ERR_ARG;
     }
     int rc = set_device(transfers[0].from);
     if (rc)

@@ -20,7 +20,7 @@ def test_driver_is_built_and_links():
 @pytest.mark.gpu
 @pytest.mark.parametrize("prec,dtype", [("f32", np.float32), ("f64", np.float64)])
 @pytest.mark.parametrize("split,fused_x", [((1, 1, 1), False), ((2, 1, 2), False),
-                                           ((5, 1, 1), True)])
+                                           ((2, 2, 2), False), ((5, 1, 1), True)])
 def test_driver_matches_oracle(oracle_cpu, tmp_path, prec, dtype, split, fused_x):
     """Undivided, split with the face exchange after each half-step, and
     split along x with the x faces exchanged inside the step kernels
