@@ -225,25 +225,31 @@ def _transfers(pairs):
 @pytest.mark.parametrize("prec", ["f32", "f64", "f16s", "bf16s", "f16c"])
 def test_exchange_faces_equals_single_exchanges(hip, prec):
     """lbm_exchange_faces over all the transfers of one axis (3 x 3 subdomains
-    in chains of 3 along it: 12 transfers, two launches) leaves every solver
-    exactly as lbm_exchange_face on each transfer in turn, for every axis and
-    phase; so does a solver exchanging its two faces of an axis with itself
-    (a wall reflected through the exchange)."""
+    in chains of 3 along it, of different extents along it: 12 transfers, two
+    launches) leaves every solver exactly as lbm_exchange_face on each
+    transfer in turn, for every axis and phase; so does a solver exchanging
+    its two faces of an axis with itself (a wall reflected through the
+    exchange)."""
     from cavityflow_cuda_amd._lib import check, lib
     say("exchange_faces", prec)
     rng = np.random.default_rng(53)
     dt = np.float64 if prec == "f64" else np.float32
     w = np.array([1 / 18] * 4 + [1 / 36] * 12 + [1 / 18] * 2 + [1 / 3])
-    size = (5, 4, 3)
-    cells = int(np.prod([v + 2 for v in size]))
     L = lib()
     for axis in range(3):
+        # extents along the axis differ between the solvers of a chain (1, 3
+        # and 6 interior cells), so one launch mixes row and plane pitches
+        sizes = []
+        for k in range(9):
+            sz = [5, 4, 3]
+            sz[axis] = (1, 3, 6)[k % 3]
+            sizes.append(tuple(sz))
         for phase in (0, 1):
             grids = []
-            starts = [(w[:, None] * (1 + 0.1 * rng.standard_normal((19, cells)))).astype(dt).ravel()
-                      for _ in range(9)]
+            starts = [(w[:, None] * (1 + 0.1 * rng.standard_normal(
+                (19, int(np.prod([v + 2 for v in sz])))))).astype(dt).ravel() for sz in sizes]
             for _ in range(2):
-                g = [hip.LbmSolverHIP(size, (GHOST,) * 6, prec) for _ in range(9)]
+                g = [hip.LbmSolverHIP(sz, (GHOST,) * 6, prec) for sz in sizes]
                 for sv, st in zip(g, starts):
                     sv.setDensityDistributions(st, *full_box(sv))
                 grids.append(g)
