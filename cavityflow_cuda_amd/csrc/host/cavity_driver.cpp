@@ -182,25 +182,28 @@ public:
     }
 
     // the faces in the reference order x, then y, then z; the transfers of
-    // one axis are independent and go to lbm_exchange_faces together
+    // one axis (both ways between every neighbour pair, listed once) are
+    // independent and go to lbm_exchange_faces together
     void exchange(int phase)
     {
-        for (int axis = 0; axis < 3; axis++) {
-            std::vector<lbm_face_transfer> t;
-            for (int k = 0; k < o_.split[2]; k++)
-                for (int j = 0; j < o_.split[1]; j++)
-                    for (int i = 0; i < o_.split[0]; i++) {
-                        int nb[3] = {i, j, k};
-                        nb[axis]++;
-                        if (nb[axis] >= o_.split[axis])
-                            continue;
-                        auto* a = at(i, j, k);
-                        auto* b = at(nb[0], nb[1], nb[2]);
-                        t.push_back({a->handle(), 2 * axis + 1, b->handle()});
-                        t.push_back({b->handle(), 2 * axis, a->handle()});
-                    }
-            lbm::check(lbm_exchange_faces(t.data(), (int)t.size(), phase, nullptr));
+        if (!xfers_ready_) {
+            for (int axis = 0; axis < 3; axis++)
+                for (int k = 0; k < o_.split[2]; k++)
+                    for (int j = 0; j < o_.split[1]; j++)
+                        for (int i = 0; i < o_.split[0]; i++) {
+                            int nb[3] = {i, j, k};
+                            nb[axis]++;
+                            if (nb[axis] >= o_.split[axis])
+                                continue;
+                            auto* a = at(i, j, k);
+                            auto* b = at(nb[0], nb[1], nb[2]);
+                            xfers_[axis].push_back({a->handle(), 2 * axis + 1, b->handle()});
+                            xfers_[axis].push_back({b->handle(), 2 * axis, a->handle()});
+                        }
+            xfers_ready_ = true;
         }
+        for (const auto& t : xfers_)
+            lbm::check(lbm_exchange_faces(t.data(), (int)t.size(), phase, nullptr));
     }
 
     // x faces exchanged inside the step kernels: every subdomain packs its
@@ -363,6 +366,8 @@ private:
     std::vector<void*> xbufs_;                  // fused x exchange: pack buffers
     std::vector<std::array<void*, 2>> xrecv_;   // per solver: received LEFT, RIGHT
     bool xarmed_ = false;
+    std::array<std::vector<lbm_face_transfer>, 3> xfers_;   // per axis
+    bool xfers_ready_ = false;
     long counter_ = 0;
 };
 
