@@ -242,6 +242,8 @@ class CavityController:
             self._slabs_done.close()
             self._exchange_done.close()
             self.overlap = False
+        # the transfer lists hold the solvers' raw handles
+        self._transfers = []
         for s in self.solvers.values():
             s.close()
         self.solvers = {}
