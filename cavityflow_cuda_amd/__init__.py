@@ -15,11 +15,12 @@ raises.
 """
 from ._lib import (BACK, BF16S, BOTTOM, F16C, F16S, F32, F64, FLUID, FRONT, GHOST_LAYER, LEFT,
                    OBSTACLE, RIGHT, TOP, VELOCITY_INJECTION, LbmError)
-from .solver import LbmSolverHIP, compute_parameters
+from .solver import LbmSolverHIP, compute_parameters, exchange_faces, face_transfers
 from .controller import CavityController, cavity_parameters
 
 __all__ = [
     "LbmSolverHIP", "CavityController", "cavity_parameters", "compute_parameters", "LbmError",
+    "exchange_faces", "face_transfers",
     "OBSTACLE", "FLUID", "VELOCITY_INJECTION", "GHOST_LAYER", "LEFT", "RIGHT", "BOTTOM", "TOP",
     "BACK", "FRONT", "F32", "F64", "F16S", "BF16S", "F16C",
 ]

@@ -45,7 +45,8 @@ import numpy as np
 
 from . import _lib
 from ._lib import check, lib
-from .solver import LbmSolverHIP, compute_parameters, host_dtype, precision_code, _stream
+from .solver import (LbmSolverHIP, compute_parameters, exchange_faces, face_transfers, host_dtype,
+                     precision_code)
 
 
 def cavity_parameters(size, u_lid, reynolds, precision="f32", max_velocity=0.1,
@@ -116,10 +117,10 @@ class CavityController:
                 nb[axis] += 1
                 b = self.solvers.get(tuple(nb))
                 if b is not None:
-                    t.append((a.handle.value, 2 * axis + 1, b.handle.value))
-                    t.append((b.handle.value, 2 * axis, a.handle.value))
+                    t.append((a, 2 * axis + 1, b))
+                    t.append((b, 2 * axis, a))
             if t:
-                self._transfers.append(((_lib.lbm_face_transfer * len(t))(*t), len(t)))
+                self._transfers.append(face_transfers(t))
         self.overlap = bool(overlap) and len(self.solvers) > 1
         # x slabs may be thicker than the reference's two cells: a slab two cells
         # wide touches one cache line per row and plane for 8 bytes of data
@@ -295,10 +296,9 @@ class CavityController:
         self._exchange_done.wait_on(self.stream)
 
     def _exchange(self, phase: int, stream=None):
-        L = lib()
-        st = _stream(self.stream if stream is None else stream)
-        for arr, n in self._transfers:   # x, then y, then z
-            check(L.lbm_exchange_faces(arr, n, phase, st))
+        st = self.stream if stream is None else stream
+        for arr in self._transfers:   # x, then y, then z
+            exchange_faces(arr, phase, st)
 
     def stepAlpha(self):
         if self.overlap:

@@ -258,6 +258,13 @@ class LbmSolverHIP:
             raise LbmError(_lib.LBM_OK - 1, "unpack_face needs a device buffer")
         check(lib().lbm_unpack_face(self._h, int(direction), int(phase), ptr, _stream(stream)))
 
+    def exchange_face(self, direction, phase, to, stream=None):
+        """lbm_exchange_face: pack_face(direction, phase) of this solver and
+        unpack_face into `to` across the opposite face, in one launch with no
+        buffer (two subdomains on one device)."""
+        check(lib().lbm_exchange_face(self._h, int(direction), int(phase), to.handle,
+                                      _stream(stream)))
+
     def _step_buffers(self, left, right, what):
         ptrs, keep = [], []
         for b in (left, right):
@@ -344,6 +351,25 @@ class LbmSolverHIP:
         size = (self.size[0], 1, self.size[2])
         self.setFlags(np.full(size[0] * size[2], _lib.VELOCITY_INJECTION, np.int32), origin,
                       size)
+
+
+def face_transfers(transfers):
+    """The lbm_face_transfer array of [(from_solver, direction, to_solver), ...]
+    for exchange_faces; it holds the solvers' handles, so it is valid while
+    they are open."""
+    arr = (_lib.lbm_face_transfer * len(transfers))()
+    for k, (a, d, b) in enumerate(transfers):
+        arr[k].from_, arr[k].direction, arr[k].to = a.handle.value, int(d), b.handle.value
+    return arr
+
+
+def exchange_faces(transfers, phase, stream=None):
+    """lbm_exchange_faces: the face transfers of one axis, a list of
+    (from_solver, direction, to_solver) or a face_transfers array, in
+    ceil(n / 8) launches, with the result of exchange_face on each in turn.
+    All on one axis, device and precision, no two into the same face."""
+    arr = transfers if isinstance(transfers, ctypes.Array) else face_transfers(transfers)
+    check(lib().lbm_exchange_faces(arr, len(arr), int(phase), _stream(stream)))
 
 
 def compute_parameters(length_x, size_x, length_max, velocity, acceleration=(0.0, 0.0, 0.0),

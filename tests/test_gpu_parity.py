@@ -260,11 +260,10 @@ def test_exchange_faces_equals_single_exchanges(hip, prec):
                     pairs.append((a, 2 * axis + 1, b))
                     pairs.append((b, 2 * axis, a))
             batched, single = grids
-            check(L.lbm_exchange_faces(_transfers([(batched[a], d, batched[b]) for a, d, b in pairs]),
-                                       len(pairs), phase, None))
+            hip.exchange_faces([(batched[a], d, batched[b]) for a, d, b in pairs], phase)
             for a, d, b in pairs:
-                check(L.lbm_exchange_face(single[a].handle, d, phase, single[b].handle, None))
-            # self exchange of solver 0's two faces on the axis
+                single[a].exchange_face(d, phase, single[b])
+            # self exchange of solver 0's two faces on the axis, through the C ABI
             self_pairs = [(0, 2 * axis + 1, 0), (0, 2 * axis, 0)]
             check(L.lbm_exchange_faces(_transfers([(batched[a], d, batched[b])
                                                    for a, d, b in self_pairs]), 2, phase, None))
